@@ -1,0 +1,169 @@
+"""Throughput of the FP64 USL step on one MI355X.
+
+Workload: the square-silo discharge -- 512 x 512 cells (513^2 nodes),
+1,048,576 particles (4 x 4 per cell, seeded jitter), g_local_mu2, square_silo
+walls, dt = 2e-6.  A step is one explicit_mpm_step_usl_threaded +
+time_varying_loads over all particles, with the state resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is particle-steps per second on the GPU.
+`roofline` prices the dominant kernel's algorithmic bytes (FP64 SoA, each
+field moved once per pass that needs it) against the 8 TB/s HBM peak, with
+kernel durations from HIP events recorded on the library's stream during the
+timed steps.  `cpu_baseline` times the CPU restatement of the reference step
+(reference AoS layout, pthreads, colour lists, barriers; compiled with the
+reference's -O3 and default FP contraction) on the host's cores over a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "mpm-2d_amd"), os.path.join(REPO, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import pympm  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per unit (FP64 SoA, each field once per pass that needs it):
+#   P2G+grid kernel: particles read x y vx vy m v sxx sxy syy (72 B);
+#                    nodes: scatter writes m px py fx fy (40 B), the grid
+#                    update reads them (40 B) and writes px py fx fy (32 B)
+#   G2P+stress kernel: particles read x y vx vy m v Txx Txy Tyy Tzz gp ux uy
+#                    (104 B) and write x y vx vy v Txx Txy Tyy Tzz gp gdp ux uy
+#                    (104 B); nodes read m px py fx fy (40 B)
+P2G_BYTES_PER_PARTICLE, P2G_BYTES_PER_NODE = 72, 112
+G2P_BYTES_PER_PARTICLE, G2P_BYTES_PER_NODE = 208, 40
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_silo_512.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*%s\s+traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)" % kernel, line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def cpu_baseline(w, budget_s=15.0):
+    """The CPU restatement on all host cores, until `budget_s` of stepping."""
+    import pyoracle
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    variant = "v4" if " avx512f" in flags else "v3"
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = re.search(r"model name\s*:\s*(.*)", flags)
+    o = pyoracle.from_workload(w, variant=variant, num_threads=threads)
+    o.run(1)   # first-touch of the AoS arrays and colour lists
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 2000:
+            break
+    nact = int(o.particles()["active"].sum())
+    o.close()
+    return {"value": nact * steps / el, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"{w['name']}: {len(w['particles']['x'])} particles, {steps} steps "
+                       f"after 1 warm-up step ({el:.1f} s), oracle/usl_oracle.c x86-64-"
+                       f"{variant} build with the reference's -O3 and FP contraction, "
+                       f"{threads} pthreads"),
+            "cpu": model.group(1).strip() if model else "unknown"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the USL step is built and timed on a single "
+              "MI355X; multi-GPU sharding is a design note, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = workloads.silo()
+    npart = len(w["particles"]["x"])
+    nnodes = w["N"] * w["N"]
+    log(f"bench: {w['name']} particles={npart} nodes={nnodes} warmup={args.warmup} "
+        f"steps={args.steps}")
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
+                            g_mag=w["g_mag"], ramp_time=w["ramp_time"])
+    sim = pympm.Sim(cfg, w["particles"])
+    sim.step(args.warmup)
+    sim.sync()
+    nact0 = int(sim.active().sum())
+
+    t0 = time.perf_counter()
+    ms = sim.time_steps(args.steps)       # kernels bracketed by HIP events
+    sim.sync()
+    wall = time.perf_counter() - t0
+    nact1 = int(sim.active().sum())
+    pstep = (nact0 + nact1) / 2.0 * args.steps
+
+    p2g_ms, g2p_ms = ms[1] / args.steps, ms[2] / args.steps
+    p2g_bytes = P2G_BYTES_PER_PARTICLE * npart + P2G_BYTES_PER_NODE * nnodes
+    g2p_bytes = G2P_BYTES_PER_PARTICLE * npart + G2P_BYTES_PER_NODE * nnodes
+    kernels = {
+        "k_p2g_grid": (p2g_ms, p2g_bytes),
+        "k_g2p_stress": (g2p_ms, g2p_bytes),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    dom_ms, dom_bytes = kernels[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = p2g_bytes + g2p_bytes
+    result = {
+        "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
+        "value": pstep / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter, generated in bench.py)",
+        "config": {"workload": "square-silo discharge, 512x512 cells (513^2 nodes), "
+                               "1,048,576 particles, g_local_mu2, dt=2e-6",
+                   "particles": npart, "nodes": nnodes, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(dom), "kernel": dom,
+                     "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
+        "kernels_ms": {k: v[0] for k, v in kernels.items()},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
+                          "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    }
+    sim.close()
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle restatement, all host cores)")
+        result["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

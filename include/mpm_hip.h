@@ -1,0 +1,184 @@
+/*
+ * mpm_hip.h -- C ABI of libmpm_hip.so, the MI355X (gfx950) implementation of
+ * the explicit update-stress-last (USL) MPM step.
+ *
+ * Plain C types only: sizes, host pointers, an opaque context.  Every entry
+ * point returns an mpm_hip_status (0 = success) and never calls exit().
+ * One context = one job on one GPU, driven from one host thread; calls on
+ * one context are not thread-safe (the reference drives its step from the
+ * serial section of its barrier, src/main.c:778-787, and so does the drop-in
+ * libmpm in mpm-2d_amd/host/).
+ *
+ * Which reference interface each entry point stands behind:
+ *   mpm_hip_create          mpm_init (libmpm/process_usl.c:68-280) plus the
+ *                           driver's set-up before the loop
+ *                           (src/main.c:742-823): dt, material_init
+ *                           (materialsrc/g_local_mu2.c:102-151),
+ *                           initial_loads (libmpm/loading.c:21-31),
+ *                           bc_validate (bcsrc/box.c:42-96,
+ *                           bcsrc/square_silo.c:32-48)
+ *   mpm_hip_set_particles   the particle array handed to mpm_init
+ *                           (fields of src/reader.c:40-119)
+ *   mpm_hip_step            n iterations of mpm_run_until's loop body
+ *                           (src/main.c:946-976): explicit_mpm_step_usl_threaded
+ *                           (libmpm/process_usl.c:465-592) followed by
+ *                           time_varying_loads (libmpm/loading.c:34-54)
+ *   mpm_hip_set_bc_mask     the Dirichlet arrays a BC plugin's
+ *                           bc_time_varying writes (u_dirichlet,
+ *                           u_dirichlet_mask, node_number_override,
+ *                           libmpm/process.h:175-182), reduced to the zeroing
+ *                           that bc_momentum/bc_force apply (box.c:183-224)
+ *   mpm_hip_get_* / set_*   the host reads of particle state between steps
+ *                           (src/writer.c:169-201, 266-380)
+ *   mpm_hip_destroy         mpm_cleanup (libmpm/process_usl.c:974-1004)
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mpm_hip_status {
+    MPM_HIP_OK = 0,
+    MPM_HIP_ERR_INVALID = -1,        /* bad argument */
+    MPM_HIP_ERR_DEVICE = -2,         /* a HIP runtime call failed */
+    MPM_HIP_ERR_NO_DEVICE = -3,      /* no usable gfx950 device */
+    MPM_HIP_ERR_BIN_OVERFLOW = -4,   /* a cell held more particles than the
+                                        bin capacity; results since the
+                                        last good sync are invalid */
+    MPM_HIP_ERR_ALLOC = -5
+} mpm_hip_status;
+
+/* Boundary-condition evaluation on the device. */
+typedef enum mpm_hip_bc_kind {
+    MPM_BC_NONE = 0,        /* no Dirichlet nodes */
+    MPM_BC_BOX = 1,         /* bcsrc/box.c: fp64 {x0, y0, w, h},
+                               int {left, bottom, right, top} friction bits
+                               (1 = fix X, 2 = fix Y) */
+    MPM_BC_SQUARE_SILO = 2, /* bcsrc/square_silo.c: fp64 {hole_radius,
+                               open_time, wall_frac} */
+    MPM_BC_MASK = 3         /* host-supplied per-node mask, see
+                               mpm_hip_set_bc_mask */
+} mpm_hip_bc_kind;
+
+/* g_local_mu2 constants (materialsrc/g_local_mu2.c:30-41, 139-140, 247).
+ * The reference hard-codes all but E and nu. */
+typedef struct mpm_hip_material {
+    double E, nu;
+    double mu_s;            /* reference 0.3819 */
+    double mu_2;            /* reference 0.6435 */
+    double I_0;             /* reference 0.278 */
+    double grains_d;        /* reference 0.001 * 5 */
+    double grains_rho;      /* reference 2450 */
+    double rho_crit;        /* reference 1485 */
+} mpm_hip_material;
+
+/* Flags */
+#define MPM_HIP_STORE_AUX 0x1  /* also store the per-step strain rate
+                                  (exx_t, exy_t, eyy_t, wxy_t), particle
+                                  acceleration (x_tt, y_tt) and local
+                                  coordinates (xl, yl); off by default, they
+                                  are not read by the next step */
+
+typedef struct mpm_hip_config {
+    int N;                  /* nodes per side; grid covers [0,1]^2 */
+    double h;               /* node spacing, len / (N - 1) */
+    double dt;
+    double t0;              /* job->t when the loop starts */
+    double g_mag;           /* loading.c G_MAG, reference 9.8 */
+    double ramp_time;       /* loading.c RAMP_TIME, reference 0.1 */
+    mpm_hip_material material;
+    int bc_kind;            /* mpm_hip_bc_kind */
+    double bc_fp64[4];
+    int bc_int[4];
+    int bin_capacity;       /* max particles per cell; 0 = 32 */
+    int flags;
+    int device;             /* HIP device ordinal */
+} mpm_hip_config;
+
+typedef struct mpm_hip_ctx mpm_hip_ctx;
+
+/* Particle fields for mpm_hip_set_field / mpm_hip_get_field. */
+typedef enum mpm_hip_field {
+    MPM_F_X = 0, MPM_F_Y, MPM_F_VX, MPM_F_VY, MPM_F_M, MPM_F_V,
+    MPM_F_TXX, MPM_F_TXY, MPM_F_TYY, MPM_F_TZZ,
+    MPM_F_GAMMAP,           /* state[9] */
+    MPM_F_GAMMADOTP,        /* state[10] */
+    MPM_F_UX, MPM_F_UY,
+    /* auxiliary outputs, valid only with MPM_HIP_STORE_AUX */
+    MPM_F_EXX, MPM_F_EXY, MPM_F_EYY, MPM_F_WXY, MPM_F_AX, MPM_F_AY,
+    MPM_F_XL, MPM_F_YL,
+    MPM_F_COUNT
+} mpm_hip_field;
+
+/* Node fields for mpm_hip_get_node_field (values after the last step). */
+typedef enum mpm_hip_node_field {
+    MPM_N_M = 0, MPM_N_MX, MPM_N_MY, MPM_N_FX, MPM_N_FY, MPM_N_VX, MPM_N_VY,
+    MPM_N_COUNT
+} mpm_hip_node_field;
+
+void mpm_hip_default_config(mpm_hip_config *cfg);
+
+/* Allocates device state for `num_particles` particles on an N x N grid. */
+int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles,
+                   mpm_hip_ctx **out);
+
+/* Initial particle state as read from a particle file: mass, volume,
+ * position, velocity, in-plane stress (NULL = zero).  Performs mpm_init's
+ * element assignment and deactivation, g_local_mu2's material_init
+ * (T from s, state zeroed) and initial_loads (zero body force), and bins the
+ * particles for the first step. */
+int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
+                          const double *x, const double *y,
+                          const double *vx, const double *vy,
+                          const double *sxx, const double *sxy,
+                          const double *syy);
+
+/* Overwrites one particle field (host array of num_particles doubles).
+ * Writing X or Y re-bins the particles. */
+int mpm_hip_set_field(mpm_hip_ctx *ctx, int field, const double *host);
+int mpm_hip_get_field(mpm_hip_ctx *ctx, int field, double *host);
+/* job->active and job->in_element as the reference would hold them after
+ * the last completed step. */
+int mpm_hip_get_active(mpm_hip_ctx *ctx, int *host);
+int mpm_hip_get_node_field(mpm_hip_ctx *ctx, int field, double *host);
+
+/* MPM_BC_MASK: per node, bit 0 = X DOF fixed (zero x momentum and force),
+ * bit 1 = Y DOF fixed.  Takes effect from the next step. */
+int mpm_hip_set_bc_mask(mpm_hip_ctx *ctx, const uint8_t *mask);
+
+/* Enqueues n USL steps on the context's stream; returns without waiting. */
+int mpm_hip_step(mpm_hip_ctx *ctx, int n);
+/* Waits for queued steps and reports device-side errors. */
+int mpm_hip_sync(mpm_hip_ctx *ctx);
+
+/* Simulation time and body force as the device holds them (waits). */
+int mpm_hip_get_time(mpm_hip_ctx *ctx, double *t, double *by);
+/* Number of element changes seen since creation (the reference logs one
+ * line per change, libmpm/process_usl.c:651-663). */
+int mpm_hip_get_crossings(mpm_hip_ctx *ctx, long long *count);
+
+/* Runs n steps and times them with HIP events recorded on the context's
+ * stream.  ms[0] = whole span, ms[1] = sum of P2G+grid kernel times,
+ * ms[2] = sum of G2P+stress kernel times (per-kernel events). */
+int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms);
+
+int mpm_hip_destroy(mpm_hip_ctx *ctx);
+
+const char *mpm_hip_status_string(int status);
+
+/* Test hooks: the library's exp() on the host, and device evaluation of
+ * exp, division and square root over n inputs (op 0: exp(a), 1: a / b,
+ * 2: sqrt(a)) to check bitwise agreement with the host. */
+double mpm_hip_exp_host(double x);
+int mpm_hip_eval_ops(int op, const double *a, const double *b, double *out,
+                     size_t n, int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_HIP_H */

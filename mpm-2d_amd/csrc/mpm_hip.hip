@@ -1,0 +1,1177 @@
+// mpm_hip.hip -- gfx950 implementation of the explicit USL MPM step behind
+// the C ABI declared in include/mpm_hip.h.
+//
+// One step is two kernels on one stream:
+//
+//   k_p2g_grid   one thread per grid node, 16x16 nodes per workgroup.  The
+//                workgroup stages the particle lists of the 17x17 cells
+//                around its tile in LDS and sorts each list by particle index.
+//                Each thread then folds, for its own node, the contributions
+//                of the four adjacent cells in colour order 0..3 and, inside
+//                a cell, of its particles in ascending index -- the exact
+//                summation order of the reference's coloured, barrier-
+//                separated scatter (libmpm/process_usl.c:812-910,
+//                libmpm/map.c:85-104) -- so the node sums are bitwise those
+//                of the CPU step, with no atomics.  The same thread applies
+//                the boundary conditions (bc_momentum/bc_force,
+//                bcsrc/box.c:183-224) and the grid update (move_grid_split,
+//                libmpm/process_usl.c:596-625) and writes one 64-byte node
+//                record.
+//
+//   k_g2p_stress one thread per particle: G2P (process_usl.c:914-957),
+//                strain rate (764-808), volume update (961-970), the
+//                g_local_mu2 stress update (materialsrc/g_local_mu2.c:
+//                197-307), the next step's element (process_usl.c:43-44,
+//                631-674) and insertion into the next step's cell bins.
+//
+// Shape functions (process_usl.c:727-760, particle.c:14-30,
+// interpolate.c:23-65) are recomputed where needed from the same inputs with
+// the same operations, so both kernels see identical values.  Everything is
+// FP64 and compiled with -ffp-contract=off; exp is the IEEE-only mpm_exp.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "mpm_exp.h"
+#include "mpm_hip.h"
+
+namespace {
+
+constexpr double kTol = 1e-10;     // TOL in process_usl.c:25 and particle.c:10
+constexpr int kTile = 16;          // nodes per tile side in k_p2g_grid
+constexpr int kRegion = kTile + 1; // cells per staged region side
+constexpr int kP2GThreads = kTile * kTile;
+constexpr int kG2PThreads = 256;
+
+enum : int { kStatusBinOverflow = 1 };
+
+struct DevState {
+    double t;           // job->t
+    double bx, by;      // body force of the next step (loading.c)
+    double t0;          // job->step_start_time
+    unsigned long long crossings;
+    int status;
+    int pad;
+};
+
+struct GridParams {
+    int N;              // nodes per side
+    int Ne;             // cells per side
+    int ncells;
+    int cap;            // bin capacity
+    double h;
+    double inv_h;       // grad_tent's 1.0 / h
+    double dt;
+    double g_mag, ramp;
+    int bc_kind;
+    int left_col, right_col, bottom_row, top_row;
+    int left_fric, bottom_fric, right_fric, top_fric;
+    int wall_col;
+    double hole_radius, open_time;
+};
+
+struct Material {
+    double G, two_G, lambda;
+    double mu_s, mu_2, I_0, grains_d, grains_rho, rho_crit;
+};
+
+struct Particles {
+    double *x, *y, *vx, *vy, *m, *v;
+    double *txx, *txy, *tyy, *tzz, *gp, *gdp, *ux, *uy;
+    double *exx, *exy, *eyy, *wxy, *ax, *ay, *xl, *yl;   // auxiliary
+    int *elem;          // element of the next step, -1 outside the grid
+    int *active;
+};
+
+struct alignas(64) NodeRec {
+    double m, mx, my, fx, fy, vx, vy, pad;
+};
+
+// ---------------------------------------------------------------------------
+// Point operations shared by both kernels.
+// ---------------------------------------------------------------------------
+
+// WHICH_ELEMENT4 (process_usl.c:43-44).  An index at or beyond the element
+// count (an ulp-level rounding case the reference reads out of bounds for)
+// is treated as outside.
+__device__ __forceinline__ int which_element(double x, double y, const GridParams &g)
+{
+    if (x < 1.0 && x >= 0.0 && y < 1.0 && y >= 0.0) {
+        double e = floor(x / g.h) + floor(y / g.h) * (double)g.Ne;
+        int ei = (int)e;
+        return ei < g.ncells ? ei : -1;
+    }
+    return -1;
+}
+
+// global_to_local_coords' snapping of |xl| in (1, 1 + TOL) to +-1
+// (particle.c:14-30).
+__device__ __forceinline__ double snap_local(double l)
+{
+    const double sgn = copysign(1.0, l);
+    if ((sgn * l) > 1.0 && ((sgn * l) - 1.0) < kTol) {
+        return sgn;
+    }
+    return l;
+}
+
+struct Shape {
+    double s[4], b1[4], b2[4];
+    double xl, yl;
+};
+
+// tent / grad_tent (interpolate.c:23-65) for the cell with lower-left node
+// (cc, rr); node order BL, BR, TR, TL.
+__device__ __forceinline__ void shape_functions(double x, double y, int cc, int rr,
+                                                const GridParams &g, Shape &sh)
+{
+    const double xn = (double)cc * g.h;
+    const double yn = (double)rr * g.h;
+    const double xl = snap_local((x - xn) / g.h);
+    const double yl = snap_local((y - yn) / g.h);
+    const double ox = 1.0 - xl, oy = 1.0 - yl;
+    sh.s[0] = ox * oy;
+    sh.s[1] = xl * oy;
+    sh.s[2] = xl * yl;
+    sh.s[3] = ox * yl;
+    sh.b1[0] = -oy * g.inv_h;
+    sh.b1[1] = oy * g.inv_h;
+    sh.b1[2] = yl * g.inv_h;
+    sh.b1[3] = -yl * g.inv_h;
+    sh.b2[0] = -ox * g.inv_h;
+    sh.b2[1] = -xl * g.inv_h;
+    sh.b2[2] = xl * g.inv_h;
+    sh.b2[3] = ox * g.inv_h;
+    sh.xl = xl;
+    sh.yl = yl;
+}
+
+// Which DOFs of node (i, j) the boundary plugin fixes: bit 0 = X, bit 1 = Y.
+//   box         bcsrc/box.c:106-169
+//   square_silo bcsrc/square_silo.c:58-92 (trapdoor open once t > open_time)
+__device__ __forceinline__ unsigned bc_fixed(const GridParams &g, int i, int j,
+                                             double t_now, const uint8_t *mask)
+{
+    unsigned f = 0;
+    switch (g.bc_kind) {
+    case MPM_BC_BOX:
+        if (i >= g.left_col && i <= g.right_col) {
+            if (j == g.bottom_row) f |= (unsigned)g.bottom_fric & 3u;
+            if (j == g.top_row) f |= (unsigned)g.top_fric & 3u;
+        }
+        if (j >= g.bottom_row && j <= g.top_row) {
+            if (i == g.left_col) f |= (unsigned)g.left_fric & 3u;
+            if (i == g.right_col) f |= (unsigned)g.right_fric & 3u;
+        }
+        break;
+    case MPM_BC_SQUARE_SILO:
+        if (j == 0) {
+            const double xn = (double)i * g.h;
+            if (!(xn <= g.hole_radius && t_now > g.open_time)) f |= 3u;
+        }
+        if (i == 0 || i == g.wall_col) f |= 1u;
+        break;
+    case MPM_BC_MASK:
+        f = mask[(size_t)j * g.N + i] & 3u;
+        break;
+    default:
+        break;
+    }
+    return f;
+}
+
+// g_local_mu2 calculate_stress_threaded body (g_local_mu2.c:197-307) on the
+// four independent components of the plane-strain stress.  The reference
+// carries full 3x3 tensors whose XZ/YZ/ZX/ZY entries stay zero and whose
+// YX entry stays equal to XY; dropping those terms changes no value (only
+// the sign of an exact zero could differ).  Every remaining operation is
+// the reference's, in its order.
+__device__ __forceinline__ void g_local_mu2_update(double exx, double exy, double eyy,
+                                                   double wxy, double m, double v,
+                                                   double dt, const Material &mt,
+                                                   double &txx, double &txy,
+                                                   double &tyy, double &tzz,
+                                                   double &gp, double &gdp)
+{
+    // L, W = skw(L), D = sym(L)
+    const double lxy = exy + wxy;
+    const double lyx = exy - wxy;
+    const double w = 0.5 * (lxy - lyx);
+    const double dxx = 0.5 * (exx + exx);
+    const double dxy = 0.5 * (lxy + lyx);
+    const double dyy = 0.5 * (eyy + eyy);
+    const double trD = (dxx + dyy) + 0.0;
+    // Jaumann term JS = W T - T W
+    const double a = w * txy;
+    const double jsxx = a + a;
+    const double jsxy = w * tyy - w * txx;
+    const double jsyy = -(a + a);
+    // trial stress Ttr = T + dt (2G D + lambda tr(D) I + JS)
+    const double ltr = mt.lambda * trD;
+    double rxx = ((dxx * mt.two_G + ltr) + jsxx) * dt + txx;
+    double rxy = ((dxy * mt.two_G) + jsxy) * dt + txy;
+    double ryy = ((dyy * mt.two_G + ltr) + jsyy) * dt + tyy;
+    double rzz = ltr * dt + tzz;
+    // deviator, pressure, equivalent shear stress
+    const double c = ((rxx + ryy) + rzz) / 3.0;
+    const double d_xx = rxx - c, d_yy = ryy - c, d_zz = rzz - c;
+    const double p_tr = c * -1.0;
+    double tau2 = d_xx * d_xx;
+    tau2 += rxy * rxy;
+    tau2 += rxy * rxy;
+    tau2 += d_yy * d_yy;
+    tau2 += d_zz * d_zz;
+    const double tau_tr = sqrt(0.5 * tau2);
+
+    const int density_flag = (m / v) < mt.rho_crit;
+    double nup_tau;
+    if (density_flag || p_tr <= 0.0) {
+        nup_tau = 0.0;
+        txx = 0.0;
+        txy = 0.0;
+        tyy = 0.0;
+        tzz = 0.0;
+    } else if (p_tr > 0.0) {
+        const double S0 = mt.mu_s * p_tr;
+        double tau_tau, scale;
+        if (tau_tr <= S0) {
+            tau_tau = tau_tr;
+            scale = 1.0;
+        } else {
+            const double S2 = mt.mu_2 * p_tr;
+            const double alpha = mt.G * mt.I_0 * dt * sqrt(p_tr / mt.grains_rho) / mt.grains_d;
+            const double B = -(S2 + tau_tr + alpha);
+            const double H = S2 * tau_tr + S0 * alpha;
+            // negative_root(1, B, H) (g_local_mu2.c:90-99)
+            if (B > 0.0) {
+                tau_tau = (-B - sqrt(B * B - 4.0 * H)) / 2.0;
+            } else {
+                tau_tau = (2.0 * H) / (-B + sqrt(B * B - 4.0 * H));
+            }
+            scale = tau_tau / tau_tr;
+        }
+        nup_tau = ((tau_tr - tau_tau) / mt.G) / dt;
+        txx = d_xx * scale - p_tr;
+        txy = rxy * scale;
+        tyy = d_yy * scale - p_tr;
+        tzz = d_zz * scale - p_tr;
+    } else {
+        nup_tau = 0.0;   // NaN pressure: stress left unchanged
+    }
+    gp += nup_tau * dt;
+    gdp = nup_tau;
+}
+
+__device__ __forceinline__ double gravity(double t, double t0, double g_mag, double ramp)
+{
+    // time_varying_loads (loading.c:34-54)
+    if ((ramp + t0) > t) {
+        return -g_mag * ((t - t0) / ramp);
+    }
+    return -g_mag;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Element assignment + binning of all particles (first step, or after the
+// host rewrote positions).  init != 0 applies mpm_init's deactivation of
+// particles outside the grid (process_usl.c:236-244).
+__global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
+    Particles P, int *__restrict__ cnt, int *__restrict__ bkt, GridParams g,
+    DevState *__restrict__ st, int np, int init)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) {
+        return;
+    }
+    if (!init && !P.active[p]) {
+        return;
+    }
+    const int e = which_element(P.x[p], P.y[p], g);
+    P.elem[p] = e;
+    if (init) {
+        P.active[p] = e >= 0;
+    }
+    if (e >= 0) {
+        const int r = atomicAdd(&cnt[e], 1);
+        if (r < g.cap) {
+            bkt[(size_t)e * g.cap + r] = p;
+        } else {
+            atomicOr(&st->status, kStatusBinOverflow);
+        }
+    }
+}
+
+template <int CAP>
+__global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
+    Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
+    const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
+    int tiles_x, int ntiles)
+{
+    __shared__ int s_cnt[kRegion * kRegion];
+    __shared__ int s_lst[kRegion * kRegion * CAP];
+
+    // Consecutive tiles go to the same XCD (workgroups are dealt to the 8
+    // XCDs round-robin), so neighbouring tiles share that XCD's L2.
+    const int per_xcd = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= ntiles) {
+        return;
+    }
+    const int tx0 = (tile % tiles_x) * kTile;
+    const int ty0 = (tile / tiles_x) * kTile;
+    const int tid = threadIdx.x;
+    const int cx0 = tx0 - 1, cy0 = ty0 - 1;   // staged cell region origin
+
+    // This tile owns cells (tx0.., ty0..): clear their next-step bin counts.
+    {
+        const int cc = tx0 + (tid % kTile), rr = ty0 + (tid / kTile);
+        if (cc < g.Ne && rr < g.Ne) {
+            cnt_next[rr * g.Ne + cc] = 0;
+        }
+    }
+
+    // Stage list lengths, then list entries.
+    for (int cl = tid; cl < kRegion * kRegion; cl += kP2GThreads) {
+        const int cc = cx0 + cl % kRegion, rr = cy0 + cl / kRegion;
+        int n = 0;
+        if (cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
+            n = cnt_cur[rr * g.Ne + cc];
+            n = n < CAP ? n : CAP;
+        }
+        s_cnt[cl] = n;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kRegion * kRegion * CAP; idx += kP2GThreads) {
+        const int cl = idx / CAP, k = idx % CAP;
+        if (k < s_cnt[cl]) {
+            const int cc = cx0 + cl % kRegion, rr = cy0 + cl / kRegion;
+            s_lst[idx] = bkt_cur[(size_t)(rr * g.Ne + cc) * CAP + k];
+        }
+    }
+    __syncthreads();
+    // Sort every list by particle index (insertion sort; lists arrive nearly
+    // sorted because lanes of one wave insert in order).
+    for (int cl = tid; cl < kRegion * kRegion; cl += kP2GThreads) {
+        int *a = &s_lst[cl * CAP];
+        const int n = s_cnt[cl];
+        for (int k = 1; k < n; ++k) {
+            const int key = a[k];
+            int q = k - 1;
+            while (q >= 0 && a[q] > key) {
+                a[q + 1] = a[q];
+                --q;
+            }
+            a[q + 1] = key;
+        }
+    }
+    __syncthreads();
+
+    const int i = tx0 + (tid % kTile);
+    const int j = ty0 + (tid / kTile);
+    if (i >= g.N || j >= g.N) {
+        return;
+    }
+    const double bx = st->bx, by = st->by;
+    double am = 0.0, amx = 0.0, amy = 0.0, afx = 0.0, afy = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        // the adjacent cell of colour c = 2 * (row % 2) + (col % 2)
+        const int cc = i - ((i ^ c) & 1);
+        const int rr = j - ((j ^ (c >> 1)) & 1);
+        if (cc < 0 || rr < 0 || cc >= g.Ne || rr >= g.Ne) {
+            continue;
+        }
+        const int dx = i - cc, dy = j - rr;      // node's corner in that cell
+        const int cl = (cc - cx0) + (rr - cy0) * kRegion;
+        const int n = s_cnt[cl];
+        const double xn = (double)cc * g.h, yn = (double)rr * g.h;
+        for (int k = 0; k < n; ++k) {
+            const int p = s_lst[cl * CAP + k];
+            const double x = P.x[p], y = P.y[p];
+            const double mp = P.m[p], vp = P.v[p];
+            const double vx = P.vx[p], vy = P.vy[p];
+            const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
+            const double xl = snap_local((x - xn) / g.h);
+            const double yl = snap_local((y - yn) / g.h);
+            const double X = dx ? xl : 1.0 - xl;
+            const double Y = dy ? yl : 1.0 - yl;
+            const double s = X * Y;
+            const double b1 = (dx ? Y : -Y) * g.inv_h;
+            const double b2 = (dy ? X : -X) * g.inv_h;
+            am += s * mp;
+            amx += s * (vx * mp);
+            amy += s * (vy * mp);
+            afx += s * (bx * mp);
+            afx += b1 * (-sxx * vp);
+            afx += b2 * (-sxy * vp);
+            afy += s * (by * mp);
+            afy += b1 * (-sxy * vp);
+            afy += b2 * (-syy * vp);
+        }
+    }
+
+    // bc_momentum then bc_force, then move_grid_split.
+    const unsigned f = bc_fixed(g, i, j, st->t + g.dt, mask);
+    if (f & 1u) amx = 0.0;
+    if (f & 2u) amy = 0.0;
+    if (f & 1u) afx = 0.0;
+    if (f & 2u) afy = 0.0;
+    double nvx = 0.0, nvy = 0.0;
+    if (am > kTol) {
+        amx += g.dt * afx;
+        amy += g.dt * afy;
+        nvx = amx / am;
+        nvy = amy / am;
+    }
+    NodeRec r;
+    r.m = am;
+    r.mx = amx;
+    r.my = amy;
+    r.fx = afx;
+    r.fy = afy;
+    r.vx = nvx;
+    r.vy = nvy;
+    r.pad = 0.0;
+    nodes[(size_t)j * g.N + i] = r;
+}
+
+__global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
+    Particles P, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
+    int *__restrict__ bkt_next, GridParams g, Material mt, DevState *__restrict__ st,
+    int np, int store_aux)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        // t += dt (process_usl.c:522), then time_varying_loads for the next
+        // step (src/main.c:975).  No thread of this kernel reads *st.
+        const double t = st->t + g.dt;
+        st->t = t;
+        st->bx = 0.0;
+        st->by = gravity(t, st->t0, g.g_mag, g.ramp);
+    }
+    if (p >= np || !P.active[p]) {
+        return;
+    }
+    const int e = P.elem[p];
+    if (e < 0) {
+        // left the grid during the previous step: deactivated at the start
+        // of this one (process_usl.c:660-665)
+        P.active[p] = 0;
+        return;
+    }
+    const int cc = e % g.Ne, rr = e / g.Ne;
+    const double dt = g.dt;
+    double x = P.x[p], y = P.y[p];
+    Shape sh;
+    shape_functions(x, y, cc, rr, g, sh);
+
+    const size_t n0 = (size_t)rr * g.N + cc;
+    const size_t nid[4] = {n0, n0 + 1, n0 + g.N + 1, n0 + g.N};
+    NodeRec nd[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nd[q] = nodes[nid[q]];
+    }
+
+    // move_particles_explicit_usl_split
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (nd[q].m < kTol) continue;
+        ax += ((sh.s[q] * nd[q].fx) / nd[q].m);
+        ay += ((sh.s[q] * nd[q].fy) / nd[q].m);
+    }
+    double vx = P.vx[p], vy = P.vy[p];
+    vx += dt * ax;
+    vy += dt * ay;
+    double dux = 0.0, duy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (nd[q].m < kTol) continue;
+        dux += ((sh.s[q] * nd[q].mx) / nd[q].m);
+        duy += ((sh.s[q] * nd[q].my) / nd[q].m);
+    }
+    dux *= dt;
+    duy *= dt;
+    x += dux;
+    y += duy;
+    const double ux = P.ux[p] + dux;
+    const double uy = P.uy[p] + duy;
+
+    // calculate_strainrate_split (SMEAR4 folds left to right)
+    const double exx = 1.0 * (sh.b1[0] * nd[0].vx + sh.b1[1] * nd[1].vx
+                              + sh.b1[2] * nd[2].vx + sh.b1[3] * nd[3].vx);
+    const double eyy = 1.0 * (sh.b2[0] * nd[0].vy + sh.b2[1] * nd[1].vy
+                              + sh.b2[2] * nd[2].vy + sh.b2[3] * nd[3].vy);
+    const double dx_tdy = 1.0 * (sh.b2[0] * nd[0].vx + sh.b2[1] * nd[1].vx
+                                 + sh.b2[2] * nd[2].vx + sh.b2[3] * nd[3].vx);
+    const double dy_tdx = 1.0 * (sh.b1[0] * nd[0].vy + sh.b1[1] * nd[1].vy
+                                 + sh.b1[2] * nd[2].vy + sh.b1[3] * nd[3].vy);
+    const double exy = 0.5 * (dx_tdy + dy_tdx);
+    const double wxy = 0.5 * (dx_tdy - dy_tdx);
+
+    // update_particle_densities_split
+    const double mp = P.m[p];
+    const double v = P.v[p] * mpm_exp(dt * (exx + eyy));
+
+    double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
+    double gp = P.gp[p], gdp;
+    g_local_mu2_update(exx, exy, eyy, wxy, mp, v, dt, mt, txx, txy, tyy, tzz, gp, gdp);
+
+    P.x[p] = x;
+    P.y[p] = y;
+    P.vx[p] = vx;
+    P.vy[p] = vy;
+    P.v[p] = v;
+    P.txx[p] = txx;
+    P.txy[p] = txy;
+    P.tyy[p] = tyy;
+    P.tzz[p] = tzz;
+    P.gp[p] = gp;
+    P.gdp[p] = gdp;
+    P.ux[p] = ux;
+    P.uy[p] = uy;
+    if (store_aux) {
+        P.exx[p] = exx;
+        P.exy[p] = exy;
+        P.eyy[p] = eyy;
+        P.wxy[p] = wxy;
+        P.ax[p] = ax;
+        P.ay[p] = ay;
+        P.xl[p] = sh.xl;
+        P.yl[p] = sh.yl;
+    }
+
+    // The next step's create_particle_to_element_map, and its bin.
+    const int en = which_element(x, y, g);
+    P.elem[p] = en;
+    if (en != e) {
+        atomicAdd(&st->crossings, 1ull);
+    }
+    if (en >= 0) {
+        const int r = atomicAdd(&cnt_next[en], 1);
+        if (r < g.cap) {
+            bkt_next[(size_t)en * g.cap + r] = p;
+        } else {
+            atomicOr(&st->status, kStatusBinOverflow);
+        }
+    }
+}
+
+__global__ void k_eval_ops(int op, const double *a, const double *b, double *out, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) {
+        out[i] = mpm_exp(a[i]);
+    } else if (op == 1) {
+        out[i] = a[i] / b[i];
+    } else {
+        out[i] = sqrt(a[i]);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct mpm_hip_ctx {
+    mpm_hip_config cfg;
+    int device;
+    size_t np;
+    GridParams g;
+    Material mt;
+    hipStream_t stream;
+    Particles P;
+    double *pfield[MPM_F_COUNT];
+    NodeRec *nodes;
+    uint8_t *mask;
+    int *cnt[2];
+    int *bkt[2];
+    DevState *st;
+    int cur;                // bins holding the current step's lists
+    int tiles_x, ntiles, p2g_blocks;
+    bool have_particles;
+};
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t err_ = (expr);                                                  \
+        if (err_ != hipSuccess) {                                                  \
+            fprintf(stderr, "mpm_hip: %s failed: %s (%s:%d)\n", #expr,              \
+                    hipGetErrorString(err_), __FILE__, __LINE__);                  \
+            return MPM_HIP_ERR_DEVICE;                                             \
+        }                                                                          \
+    } while (0)
+
+static int use_device(const mpm_hip_ctx *ctx)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    return MPM_HIP_OK;
+}
+
+static int launch_p2g(mpm_hip_ctx *ctx)
+{
+    const int cur = ctx->cur, nxt = 1 - cur;
+    const dim3 grid(ctx->p2g_blocks), block(kP2GThreads);
+    switch (ctx->g.cap) {
+    case 16:
+        k_p2g_grid<16><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
+            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        break;
+    case 32:
+        k_p2g_grid<32><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
+            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        break;
+    default:
+        k_p2g_grid<64><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
+            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
+static int launch_g2p(mpm_hip_ctx *ctx)
+{
+    const int nxt = 1 - ctx->cur;
+    const size_t nb = ctx->np ? (ctx->np + kG2PThreads - 1) / kG2PThreads : 1;
+    k_g2p_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+        ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+        (int)ctx->np, (ctx->cfg.flags & MPM_HIP_STORE_AUX) ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    ctx->cur = nxt;
+    return MPM_HIP_OK;
+}
+
+static int rebin(mpm_hip_ctx *ctx, int init)
+{
+    HIP_TRY(hipMemsetAsync(ctx->cnt[0], 0, sizeof(int) * ctx->g.ncells, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cnt[1], 0, sizeof(int) * ctx->g.ncells, ctx->stream));
+    ctx->cur = 0;
+    if (ctx->np) {
+        const size_t nb = (ctx->np + kG2PThreads - 1) / kG2PThreads;
+        k_bin_particles<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+            ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g, ctx->st, (int)ctx->np, init);
+        HIP_TRY(hipGetLastError());
+    }
+    return MPM_HIP_OK;
+}
+
+extern "C" {
+
+void mpm_hip_default_config(mpm_hip_config *cfg)
+{
+    memset(cfg, 0, sizeof *cfg);
+    cfg->N = 65;
+    cfg->h = 1.0 / 64;
+    cfg->dt = 1e-5;
+    cfg->t0 = 0.0;
+    cfg->g_mag = 9.8;
+    cfg->ramp_time = 0.1;
+    cfg->material.E = 1e6;
+    cfg->material.nu = 0.3;
+    cfg->material.mu_s = 0.3819;
+    cfg->material.mu_2 = 0.6435;
+    cfg->material.I_0 = (0.278 / 1.0);
+    cfg->material.grains_d = (0.001 * 5);
+    cfg->material.grains_rho = 2450;
+    cfg->material.rho_crit = 1485.0f;
+    cfg->bc_kind = MPM_BC_NONE;
+    cfg->bin_capacity = 0;
+    cfg->flags = 0;
+    cfg->device = 0;
+}
+
+const char *mpm_hip_status_string(int status)
+{
+    switch (status) {
+    case MPM_HIP_OK: return "ok";
+    case MPM_HIP_ERR_INVALID: return "invalid argument";
+    case MPM_HIP_ERR_DEVICE: return "HIP runtime error";
+    case MPM_HIP_ERR_NO_DEVICE: return "no gfx950 device";
+    case MPM_HIP_ERR_BIN_OVERFLOW: return "cell bin capacity exceeded";
+    case MPM_HIP_ERR_ALLOC: return "device allocation failed";
+    default: return "unknown status";
+    }
+}
+
+int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx **out)
+{
+    if (!cfg || !out || cfg->N < 2 || !(cfg->h > 0.0) || !(cfg->dt > 0.0)
+        || num_particles > (size_t)0x7fffffff || cfg->N > 46340) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int cap = cfg->bin_capacity ? cfg->bin_capacity : 32;
+    if (cap != 16 && cap != 32 && cap != 64) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) {
+        return MPM_HIP_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess
+        || strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        return MPM_HIP_ERR_NO_DEVICE;
+    }
+
+    mpm_hip_ctx *ctx = new (std::nothrow) mpm_hip_ctx();
+    if (!ctx) {
+        return MPM_HIP_ERR_ALLOC;
+    }
+    ctx->cfg = *cfg;
+    ctx->device = cfg->device;
+    ctx->np = num_particles;
+    *out = nullptr;
+
+    GridParams &g = ctx->g;
+    g.N = cfg->N;
+    g.Ne = cfg->N - 1;
+    g.ncells = g.Ne * g.Ne;
+    g.cap = cap;
+    g.h = cfg->h;
+    g.inv_h = 1.0 / cfg->h;
+    g.dt = cfg->dt;
+    g.g_mag = cfg->g_mag;
+    g.ramp = cfg->ramp_time;
+    g.bc_kind = cfg->bc_kind;
+    if (cfg->bc_kind == MPM_BC_BOX) {
+        // bc_validate (box.c:42-96)
+        const double x0 = cfg->bc_fp64[0], y0 = cfg->bc_fp64[1];
+        const double w = cfg->bc_fp64[2], hh = cfg->bc_fp64[3];
+        if (x0 < 0 || y0 < 0 || x0 > 1.0 || y0 > 1.0 || w < 0 || hh < 0
+            || (x0 + w) > 1.0 || (y0 + hh) > 1.0) {
+            delete ctx;
+            return MPM_HIP_ERR_INVALID;
+        }
+        const size_t nm1 = (size_t)cfg->N - 1;
+        g.left_col = (int)(size_t)(x0 * nm1);
+        g.right_col = (int)(size_t)((x0 + w) * nm1);
+        g.bottom_row = (int)(size_t)(y0 * nm1);
+        g.top_row = (int)(size_t)((y0 + hh) * nm1);
+        g.left_fric = cfg->bc_int[0];
+        g.bottom_fric = cfg->bc_int[1];
+        g.right_fric = cfg->bc_int[2];
+        g.top_fric = cfg->bc_int[3];
+    } else if (cfg->bc_kind == MPM_BC_SQUARE_SILO) {
+        // bc_validate (square_silo.c:32-48)
+        if (!(cfg->bc_fp64[2] >= 0.0 && cfg->bc_fp64[2] <= 1.0)) {
+            delete ctx;
+            return MPM_HIP_ERR_INVALID;
+        }
+        g.hole_radius = cfg->bc_fp64[0];
+        g.open_time = cfg->bc_fp64[1];
+        g.wall_col = (int)(((size_t)cfg->N - 1) * cfg->bc_fp64[2]);
+    } else if (cfg->bc_kind != MPM_BC_NONE && cfg->bc_kind != MPM_BC_MASK) {
+        delete ctx;
+        return MPM_HIP_ERR_INVALID;
+    }
+
+    // material_init (g_local_mu2.c:139-142) and calculate_stress_threaded's
+    // lambda (g_local_mu2.c:187)
+    Material &mt = ctx->mt;
+    const double E = cfg->material.E, nu = cfg->material.nu;
+    mt.G = E / (2.0 * (1.0 + nu));
+    const double K = E / (3.0 * (1.0 - 2 * nu));
+    mt.two_G = 2.0 * mt.G;
+    mt.lambda = K - 2.0 * mt.G / 3.0;
+    mt.mu_s = cfg->material.mu_s;
+    mt.mu_2 = cfg->material.mu_2;
+    mt.I_0 = cfg->material.I_0;
+    mt.grains_d = cfg->material.grains_d;
+    mt.grains_rho = cfg->material.grains_rho;
+    mt.rho_crit = cfg->material.rho_crit;
+
+    ctx->tiles_x = (cfg->N + kTile - 1) / kTile;
+    ctx->ntiles = ctx->tiles_x * ctx->tiles_x;
+    ctx->p2g_blocks = ((ctx->ntiles + 7) / 8) * 8;
+
+#define ALLOC_OR_FAIL(ptr, bytes)                                                   \
+    do {                                                                            \
+        if (hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 64) != hipSuccess) {     \
+            mpm_hip_destroy(ctx);                                                   \
+            return MPM_HIP_ERR_ALLOC;                                               \
+        }                                                                           \
+    } while (0)
+
+    if (hipSetDevice(ctx->device) != hipSuccess
+        || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MPM_HIP_ERR_DEVICE;
+    }
+    const size_t np = num_particles;
+    const size_t nn = (size_t)cfg->N * cfg->N;
+    for (int f = 0; f < MPM_F_COUNT; ++f) {
+        const bool aux = f >= MPM_F_EXX;
+        if (aux && !(cfg->flags & MPM_HIP_STORE_AUX)) {
+            ctx->pfield[f] = nullptr;
+            continue;
+        }
+        ALLOC_OR_FAIL(ctx->pfield[f], np * sizeof(double));
+    }
+    Particles &P = ctx->P;
+    P.x = ctx->pfield[MPM_F_X];
+    P.y = ctx->pfield[MPM_F_Y];
+    P.vx = ctx->pfield[MPM_F_VX];
+    P.vy = ctx->pfield[MPM_F_VY];
+    P.m = ctx->pfield[MPM_F_M];
+    P.v = ctx->pfield[MPM_F_V];
+    P.txx = ctx->pfield[MPM_F_TXX];
+    P.txy = ctx->pfield[MPM_F_TXY];
+    P.tyy = ctx->pfield[MPM_F_TYY];
+    P.tzz = ctx->pfield[MPM_F_TZZ];
+    P.gp = ctx->pfield[MPM_F_GAMMAP];
+    P.gdp = ctx->pfield[MPM_F_GAMMADOTP];
+    P.ux = ctx->pfield[MPM_F_UX];
+    P.uy = ctx->pfield[MPM_F_UY];
+    P.exx = ctx->pfield[MPM_F_EXX];
+    P.exy = ctx->pfield[MPM_F_EXY];
+    P.eyy = ctx->pfield[MPM_F_EYY];
+    P.wxy = ctx->pfield[MPM_F_WXY];
+    P.ax = ctx->pfield[MPM_F_AX];
+    P.ay = ctx->pfield[MPM_F_AY];
+    P.xl = ctx->pfield[MPM_F_XL];
+    P.yl = ctx->pfield[MPM_F_YL];
+    ALLOC_OR_FAIL(P.elem, np * sizeof(int));
+    ALLOC_OR_FAIL(P.active, np * sizeof(int));
+    ALLOC_OR_FAIL(ctx->nodes, nn * sizeof(NodeRec));
+    ALLOC_OR_FAIL(ctx->mask, nn);
+    for (int b = 0; b < 2; ++b) {
+        ALLOC_OR_FAIL(ctx->cnt[b], (size_t)g.ncells * sizeof(int));
+        ALLOC_OR_FAIL(ctx->bkt[b], (size_t)g.ncells * cap * sizeof(int));
+    }
+    ALLOC_OR_FAIL(ctx->st, sizeof(DevState));
+#undef ALLOC_OR_FAIL
+
+    if (hipMemsetAsync(ctx->mask, 0, nn, ctx->stream) != hipSuccess
+        || hipMemsetAsync(ctx->nodes, 0, nn * sizeof(NodeRec), ctx->stream) != hipSuccess
+        || hipMemsetAsync(P.active, 0, np * sizeof(int), ctx->stream) != hipSuccess
+        || hipMemsetAsync(P.elem, 0xff, np * sizeof(int), ctx->stream) != hipSuccess) {
+        mpm_hip_destroy(ctx);
+        return MPM_HIP_ERR_DEVICE;
+    }
+    DevState s0;
+    memset(&s0, 0, sizeof s0);
+    s0.t = cfg->t0;
+    s0.t0 = cfg->t0;
+    if (hipMemcpyAsync(ctx->st, &s0, sizeof s0, hipMemcpyHostToDevice, ctx->stream) != hipSuccess
+        || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        mpm_hip_destroy(ctx);
+        return MPM_HIP_ERR_DEVICE;
+    }
+    *out = ctx;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
+                          const double *x, const double *y, const double *vx,
+                          const double *vy, const double *sxx, const double *sxy,
+                          const double *syy)
+{
+    if (!ctx || (ctx->np && (!m || !v || !x || !y || !vx || !vy))) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const size_t bytes = ctx->np * sizeof(double);
+    const double *src[] = {x, y, vx, vy, m, v};
+    const int dst[] = {MPM_F_X, MPM_F_Y, MPM_F_VX, MPM_F_VY, MPM_F_M, MPM_F_V};
+    if (bytes) {
+        for (int k = 0; k < 6; ++k) {
+            HIP_TRY(hipMemcpyAsync(ctx->pfield[dst[k]], src[k], bytes, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        }
+        // material_init: T from the particle stress, T[ZZ] and state zero
+        const double *s3[] = {sxx, sxy, syy};
+        const int t3[] = {MPM_F_TXX, MPM_F_TXY, MPM_F_TYY};
+        for (int k = 0; k < 3; ++k) {
+            if (s3[k]) {
+                HIP_TRY(hipMemcpyAsync(ctx->pfield[t3[k]], s3[k], bytes, hipMemcpyHostToDevice,
+                                       ctx->stream));
+            } else {
+                HIP_TRY(hipMemsetAsync(ctx->pfield[t3[k]], 0, bytes, ctx->stream));
+            }
+        }
+        const int zero[] = {MPM_F_TZZ, MPM_F_GAMMAP, MPM_F_GAMMADOTP, MPM_F_UX, MPM_F_UY};
+        for (int k = 0; k < 5; ++k) {
+            HIP_TRY(hipMemsetAsync(ctx->pfield[zero[k]], 0, bytes, ctx->stream));
+        }
+        for (int f = MPM_F_EXX; f < MPM_F_COUNT; ++f) {
+            if (ctx->pfield[f]) {
+                HIP_TRY(hipMemsetAsync(ctx->pfield[f], 0, bytes, ctx->stream));
+            }
+        }
+    }
+    DevState s0;
+    memset(&s0, 0, sizeof s0);
+    s0.t = ctx->cfg.t0;
+    s0.t0 = ctx->cfg.t0;
+    HIP_TRY(hipMemcpyAsync(ctx->st, &s0, sizeof s0, hipMemcpyHostToDevice, ctx->stream));
+    rc = rebin(ctx, 1);
+    if (rc) return rc;
+    ctx->have_particles = true;
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_set_field(mpm_hip_ctx *ctx, int field, const double *host)
+{
+    if (!ctx || field < 0 || field >= MPM_F_COUNT || !ctx->pfield[field]
+        || (ctx->np && !host)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    if (ctx->np) {
+        HIP_TRY(hipMemcpyAsync(ctx->pfield[field], host, ctx->np * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (field == MPM_F_X || field == MPM_F_Y) {
+        rc = rebin(ctx, 0);
+        if (rc) return rc;
+    }
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_get_field(mpm_hip_ctx *ctx, int field, double *host)
+{
+    if (!ctx || field < 0 || field >= MPM_F_COUNT || !ctx->pfield[field]
+        || (ctx->np && !host)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    if (ctx->np) {
+        HIP_TRY(hipMemcpyAsync(host, ctx->pfield[field], ctx->np * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_get_active(mpm_hip_ctx *ctx, int *host)
+{
+    if (!ctx || (ctx->np && !host)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    if (ctx->np) {
+        HIP_TRY(hipMemcpyAsync(host, ctx->P.active, ctx->np * sizeof(int),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_get_node_field(mpm_hip_ctx *ctx, int field, double *host)
+{
+    if (!ctx || field < 0 || field >= MPM_N_COUNT || !host) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const size_t nn = (size_t)ctx->g.N * ctx->g.N;
+    // strided copy of one double out of each 64-byte record
+    HIP_TRY(hipMemcpy2DAsync(host, sizeof(double), (const char *)ctx->nodes + field * sizeof(double),
+                             sizeof(NodeRec), sizeof(double), nn, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_set_bc_mask(mpm_hip_ctx *ctx, const uint8_t *mask)
+{
+    if (!ctx || !mask) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const size_t nn = (size_t)ctx->g.N * ctx->g.N;
+    HIP_TRY(hipMemcpyAsync(ctx->mask, mask, nn, hipMemcpyHostToDevice, ctx->stream));
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_step(mpm_hip_ctx *ctx, int n)
+{
+    if (!ctx || n < 0 || !ctx->have_particles) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    for (int s = 0; s < n; ++s) {
+        if ((rc = launch_p2g(ctx)) || (rc = launch_g2p(ctx))) {
+            return rc;
+        }
+    }
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_sync(mpm_hip_ctx *ctx)
+{
+    if (!ctx) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    int status = 0;
+    HIP_TRY(hipMemcpyAsync(&status, &ctx->st->status, sizeof(int), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (status & kStatusBinOverflow) {
+        return MPM_HIP_ERR_BIN_OVERFLOW;
+    }
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_get_time(mpm_hip_ctx *ctx, double *t, double *by)
+{
+    if (!ctx) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    DevState s;
+    HIP_TRY(hipMemcpyAsync(&s, ctx->st, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t) *t = s.t;
+    if (by) *by = s.by;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_get_crossings(mpm_hip_ctx *ctx, long long *count)
+{
+    if (!ctx || !count) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    DevState s;
+    HIP_TRY(hipMemcpyAsync(&s, ctx->st, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *count = (long long)s.crossings;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms)
+{
+    if (!ctx || n <= 0 || !ms || !ctx->have_particles) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    hipEvent_t *ev = (hipEvent_t *)malloc(sizeof(hipEvent_t) * (2 * (size_t)n + 1));
+    if (!ev) return MPM_HIP_ERR_ALLOC;
+    int created = 0;
+    for (; created < 2 * n + 1; ++created) {
+        if (hipEventCreate(&ev[created]) != hipSuccess) {
+            rc = MPM_HIP_ERR_DEVICE;
+            break;
+        }
+    }
+    if (!rc) {
+        for (int s = 0; s < n && !rc; ++s) {
+            if (hipEventRecord(ev[2 * s], ctx->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+            if (!rc) rc = launch_p2g(ctx);
+            if (!rc && hipEventRecord(ev[2 * s + 1], ctx->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+            if (!rc) rc = launch_g2p(ctx);
+        }
+        if (!rc && hipEventRecord(ev[2 * n], ctx->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+        if (!rc && hipEventSynchronize(ev[2 * n]) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    }
+    if (!rc) {
+        double p2g = 0, g2p = 0;
+        float a = 0.f;
+        for (int s = 0; s < n; ++s) {
+            hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]);
+            p2g += a;
+            hipEventElapsedTime(&a, ev[2 * s + 1], ev[2 * s + 2]);
+            g2p += a;
+        }
+        hipEventElapsedTime(&a, ev[0], ev[2 * n]);
+        ms[0] = a;
+        ms[1] = p2g;
+        ms[2] = g2p;
+    }
+    for (int k = 0; k < created; ++k) {
+        hipEventDestroy(ev[k]);
+    }
+    free(ev);
+    if (rc) return rc;
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_destroy(mpm_hip_ctx *ctx)
+{
+    if (!ctx) {
+        return MPM_HIP_OK;
+    }
+    hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+    }
+    for (int f = 0; f < MPM_F_COUNT; ++f) {
+        if (ctx->pfield[f]) hipFree(ctx->pfield[f]);
+    }
+    if (ctx->P.elem) hipFree(ctx->P.elem);
+    if (ctx->P.active) hipFree(ctx->P.active);
+    if (ctx->nodes) hipFree(ctx->nodes);
+    if (ctx->mask) hipFree(ctx->mask);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->cnt[b]) hipFree(ctx->cnt[b]);
+        if (ctx->bkt[b]) hipFree(ctx->bkt[b]);
+    }
+    if (ctx->st) hipFree(ctx->st);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MPM_HIP_OK;
+}
+
+double mpm_hip_exp_host(double x)
+{
+    return mpm_exp(x);
+}
+
+int mpm_hip_eval_ops(int op, const double *a, const double *b, double *out, size_t n,
+                     int device)
+{
+    if (op < 0 || op > 2 || !a || !out || (op == 1 && !b)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    if (n == 0) {
+        return MPM_HIP_OK;
+    }
+    HIP_TRY(hipSetDevice(device));
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    const size_t bytes = n * sizeof(double);
+    int rc = MPM_HIP_OK;
+    if (hipMalloc((void **)&da, bytes) != hipSuccess
+        || hipMalloc((void **)&db, bytes) != hipSuccess
+        || hipMalloc((void **)&dout, bytes) != hipSuccess) {
+        rc = MPM_HIP_ERR_ALLOC;
+    }
+    if (!rc && (hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess
+                || hipMemcpy(db, b ? b : a, bytes, hipMemcpyHostToDevice) != hipSuccess)) {
+        rc = MPM_HIP_ERR_DEVICE;
+    }
+    if (!rc) {
+        k_eval_ops<<<dim3((unsigned)((n + 255) / 256)), dim3(256)>>>(op, da, db, dout, n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
+            || hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = MPM_HIP_ERR_DEVICE;
+        }
+    }
+    if (da) hipFree(da);
+    if (db) hipFree(db);
+    if (dout) hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+"""ctypes binding of libmpm_hip.so (include/mpm_hip.h).
+
+Host-side harness for tests and benchmarks.  It calls the C ABI only; there
+is no CPU fallback: if the HIP library is missing or no gfx950 device is
+present, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmpm_hip.so")
+
+# mpm_hip_bc_kind
+BC_NONE, BC_BOX, BC_SQUARE_SILO, BC_MASK = 0, 1, 2, 3
+STORE_AUX = 0x1
+
+# mpm_hip_field
+FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
+          "gammap", "gammadotp", "ux", "uy",
+          "exx", "exy", "eyy", "wxy", "ax", "ay", "xl", "yl"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+AUX_FIELDS = FIELDS[14:]
+NODE_FIELDS = ["m", "mx", "my", "fx", "fy", "vx", "vy"]
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error",
+          -3: "no gfx950 device", -4: "cell bin capacity exceeded",
+          -5: "device allocation failed"}
+
+
+class MpmHipError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {STATUS.get(code, code)} ({code})")
+        self.code = code
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("E", ctypes.c_double), ("nu", ctypes.c_double),
+                ("mu_s", ctypes.c_double), ("mu_2", ctypes.c_double),
+                ("I_0", ctypes.c_double), ("grains_d", ctypes.c_double),
+                ("grains_rho", ctypes.c_double), ("rho_crit", ctypes.c_double)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("h", ctypes.c_double), ("dt", ctypes.c_double),
+                ("t0", ctypes.c_double), ("g_mag", ctypes.c_double),
+                ("ramp_time", ctypes.c_double), ("material", Material),
+                ("bc_kind", ctypes.c_int), ("bc_fp64", ctypes.c_double * 4),
+                ("bc_int", ctypes.c_int * 4), ("bin_capacity", ctypes.c_int),
+                ("flags", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Loads libmpm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, D, I, Z = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_size_t
+    lib.mpm_hip_default_config.argtypes = [ctypes.POINTER(Config)]
+    lib.mpm_hip_default_config.restype = None
+    lib.mpm_hip_create.argtypes = [ctypes.POINTER(Config), Z, ctypes.POINTER(P)]
+    lib.mpm_hip_set_particles.argtypes = [P] + [D] * 9
+    lib.mpm_hip_set_field.argtypes = [P, I, D]
+    lib.mpm_hip_get_field.argtypes = [P, I, D]
+    lib.mpm_hip_get_active.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    lib.mpm_hip_get_node_field.argtypes = [P, I, D]
+    lib.mpm_hip_set_bc_mask.argtypes = [P, ctypes.POINTER(ctypes.c_uint8)]
+    lib.mpm_hip_step.argtypes = [P, I]
+    lib.mpm_hip_sync.argtypes = [P]
+    lib.mpm_hip_get_time.argtypes = [P, D, D]
+    lib.mpm_hip_get_crossings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
+    lib.mpm_hip_time_steps.argtypes = [P, I, D]
+    lib.mpm_hip_destroy.argtypes = [P]
+    lib.mpm_hip_status_string.argtypes = [I]
+    lib.mpm_hip_status_string.restype = ctypes.c_char_p
+    lib.mpm_hip_exp_host.argtypes = [ctypes.c_double]
+    lib.mpm_hip_exp_host.restype = ctypes.c_double
+    lib.mpm_hip_eval_ops.argtypes = [I, D, D, D, Z, I]
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(code, what):
+    if code != 0:
+        raise MpmHipError(code, what)
+
+
+def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
+                g_mag=9.8, ramp_time=0.1, t0=0.0, bin_capacity=0, flags=0, device=0):
+    lib = load()
+    cfg = Config()
+    lib.mpm_hip_default_config(ctypes.byref(cfg))
+    cfg.N, cfg.h, cfg.dt, cfg.t0 = N, h, dt, t0
+    cfg.g_mag, cfg.ramp_time = g_mag, ramp_time
+    for k, val in (material or {}).items():
+        setattr(cfg.material, k, val)
+    cfg.bc_kind = bc_kind
+    for i, val in enumerate(bc_fp64):
+        cfg.bc_fp64[i] = val
+    for i, val in enumerate(bc_int):
+        cfg.bc_int[i] = val
+    cfg.bin_capacity, cfg.flags, cfg.device = bin_capacity, flags, device
+    return cfg
+
+
+class Sim:
+    """One job on one GPU: the device-resident USL state."""
+
+    def __init__(self, cfg, particles):
+        self.lib = load()
+        self.cfg = cfg
+        self.np = len(particles["x"])
+        self.ctx = ctypes.c_void_p()
+        _check(self.lib.mpm_hip_create(ctypes.byref(cfg), self.np, ctypes.byref(self.ctx)),
+               "mpm_hip_create")
+        arrs = [np.ascontiguousarray(particles[k], dtype=np.float64)
+                for k in ("m", "v", "x", "y", "vx", "vy")]
+        stress = [particles.get(k) for k in ("sxx", "sxy", "syy")]
+        stress = [None if s is None else np.ascontiguousarray(s, dtype=np.float64) for s in stress]
+        ptrs = [_dptr(a) for a in arrs] + [None if s is None else _dptr(s) for s in stress]
+        _check(self.lib.mpm_hip_set_particles(self.ctx, *ptrs), "mpm_hip_set_particles")
+
+    def step(self, n=1):
+        _check(self.lib.mpm_hip_step(self.ctx, n), "mpm_hip_step")
+
+    def sync(self):
+        _check(self.lib.mpm_hip_sync(self.ctx), "mpm_hip_sync")
+
+    def get(self, name):
+        out = np.empty(self.np, dtype=np.float64)
+        _check(self.lib.mpm_hip_get_field(self.ctx, FIELD_ID[name], _dptr(out)),
+               f"mpm_hip_get_field({name})")
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        _check(self.lib.mpm_hip_set_field(self.ctx, FIELD_ID[name], _dptr(a)),
+               f"mpm_hip_set_field({name})")
+
+    def active(self):
+        out = np.empty(self.np, dtype=np.int32)
+        _check(self.lib.mpm_hip_get_active(
+            self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "mpm_hip_get_active")
+        return out
+
+    def node(self, name):
+        nn = self.cfg.N * self.cfg.N
+        out = np.empty(nn, dtype=np.float64)
+        _check(self.lib.mpm_hip_get_node_field(self.ctx, NODE_FIELDS.index(name), _dptr(out)),
+               f"mpm_hip_get_node_field({name})")
+        return out
+
+    def set_bc_mask(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(self.lib.mpm_hip_set_bc_mask(
+            self.ctx, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "mpm_hip_set_bc_mask")
+
+    def time(self):
+        t, by = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.mpm_hip_get_time(self.ctx, ctypes.byref(t), ctypes.byref(by)),
+               "mpm_hip_get_time")
+        return t.value, by.value
+
+    def crossings(self):
+        c = ctypes.c_longlong()
+        _check(self.lib.mpm_hip_get_crossings(self.ctx, ctypes.byref(c)), "mpm_hip_get_crossings")
+        return c.value
+
+    def time_steps(self, n):
+        ms = np.zeros(3, dtype=np.float64)
+        _check(self.lib.mpm_hip_time_steps(self.ctx, n, _dptr(ms)), "mpm_hip_time_steps")
+        return ms
+
+    def state(self, names=None):
+        names = names or FIELDS[:14]
+        return {k: self.get(k) for k in names}
+
+    def close(self):
+        if self.ctx:
+            self.lib.mpm_hip_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def eval_ops(op, a, b=None, device=0):
+    """Device evaluation of mpm_exp (op 0), a / b (op 1), sqrt(a) (op 2)."""
+    lib = load()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    bb = np.ascontiguousarray(b if b is not None else a, dtype=np.float64)
+    out = np.empty_like(a)
+    _check(lib.mpm_hip_eval_ops(op, _dptr(a), _dptr(bb), _dptr(out), a.size, device),
+           "mpm_hip_eval_ops")
+    return out
+
+
+def exp_host(x):
+    return load().mpm_hip_exp_host(float(x))

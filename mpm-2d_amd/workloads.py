@@ -1,0 +1,106 @@
+"""Synthetic inputs for the USL workloads (seeded, reproducible).
+
+Particles sit on a 4 x 4 lattice per cell at ((k + 1/2) / 4) h with a uniform
+jitter of +-0.1 h per axis, so no particle starts outside its cell.  The
+jitter comes from SplitMix64 keyed by (seed, particle index, axis), so any
+subset of particles can be regenerated independently.  Volume is the
+lattice area (h/4)^2 and mass rho * volume, as the reference's particle
+generators do (generate_block_particles.py:21-62, gen_particles.m); initial
+velocity and stress are zero.  Particles are numbered cell by cell (cell
+rows bottom to top, cells left to right, then lattice rows and columns), so
+memory order starts out cell-sorted.
+
+The grid always spans [0, 1]^2 (process_usl.c:43-44 hard-codes the unit
+domain): N = cells + 1 nodes per side, h = 1 / (N - 1).
+"""
+import numpy as np
+
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15)) & M64
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M64
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M64
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, ids, stream):
+    """U[0,1) keyed by (seed, id, stream), 53-bit resolution."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0xD1342543DE82EF95)
+               + ids.astype(np.uint64) * np.uint64(4) + np.uint64(stream)) & M64
+        return (splitmix64(key) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def block(cells_x, cells_y, n_cells, seed=1, rho=1500.0, ppc_side=4, jitter=0.1,
+          cell_x0=0, cell_y0=0):
+    """Particles filling cells [cell_x0, cell_x0+cells_x) x [cell_y0, ...)."""
+    h = 1.0 / n_cells
+    k = ppc_side
+    cy, cx, b, a = np.meshgrid(np.arange(cells_y) + cell_y0, np.arange(cells_x) + cell_x0,
+                               np.arange(k), np.arange(k), indexing="ij")
+    cy, cx, b, a = (q.ravel() for q in (cy, cx, b, a))
+    ids = np.arange(cx.size, dtype=np.uint64)
+    jx = (uniform01(seed, ids, 0) - 0.5) * (2.0 * jitter) * h
+    jy = (uniform01(seed, ids, 1) - 0.5) * (2.0 * jitter) * h
+    x = (cx + (a + 0.5) / k) * h + jx
+    y = (cy + (b + 0.5) / k) * h + jy
+    v = np.full(x.size, (h / k) ** 2)
+    m = rho * v
+    z = np.zeros(x.size)
+    return {"m": m, "v": v, "x": x, "y": y, "vx": z.copy(), "vy": z.copy(),
+            "sxx": z.copy(), "sxy": z.copy(), "syy": z.copy()}
+
+
+def triangle_pile(base_cells, height_cells, n_cells, seed=3, rho=1500.0, ppc_side=4):
+    """An isosceles pile: lattice points of the base_cells-wide block that lie
+    under the line from (0, height) to (base/2, ...) mirrored about the
+    centre, centred in x."""
+    x0 = (n_cells - base_cells) // 2
+    p = block(base_cells, height_cells, n_cells, seed=seed, rho=rho, ppc_side=ppc_side,
+              cell_x0=x0)
+    h = 1.0 / n_cells
+    xc = (x0 + base_cells / 2.0) * h
+    half = base_cells / 2.0 * h
+    keep = p["y"] <= height_cells * h * (1.0 - np.abs(p["x"] - xc) / half)
+    return {k: v[keep].copy() for k, v in p.items()}
+
+
+# Material parameters named by the workloads (E, nu from the configs; the
+# reference's g_local_mu2 hard-codes the rest: materialsrc/g_local_mu2.c:30-41).
+BASELINE_MATERIAL = dict(E=1e6, nu=0.3, mu_s=0.38, mu_2=0.64, I_0=0.279,
+                         grains_d=1e-3, grains_rho=2450.0, rho_crit=1485.0)
+REFERENCE_MATERIAL = dict(E=1e6, nu=0.3, mu_s=0.3819, mu_2=0.6435, I_0=0.278,
+                          grains_d=0.001 * 5, grains_rho=2450.0, rho_crit=1485.0)
+
+BOX_FRICTION = dict(bc_kind=1, bc_fp64=(0.0, 0.0, 1.0, 1.0), bc_int=(1, 3, 1, 0))
+
+
+def column_collapse(seed=1):
+    """Granular column collapse: 64 x 64 cells, 16 x 32 cell block, 8,192
+    particles, box with frictionless walls and no-slip floor."""
+    n = 64
+    return dict(name="column_collapse_64", N=n + 1, h=1.0 / n, dt=1e-5, steps=1000,
+                material=BASELINE_MATERIAL, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
+                particles=block(16, 32, n, seed=seed))
+
+
+def silo(seed=2, cells=512):
+    """Square-silo discharge: 512 x 512 cells, 256 x 256 cell block
+    (1,048,576 particles), wall at 0.6, orifice radius 0.05 open from t=0."""
+    n = cells
+    blk = cells // 2
+    return dict(name=f"silo_{n}", N=n + 1, h=1.0 / n, dt=2e-6, steps=2000,
+                material=BASELINE_MATERIAL, g_mag=9.81, ramp_time=0.1,
+                bc_kind=2, bc_fp64=(0.05, 0.0, 0.6), bc_int=(),
+                particles=block(blk, blk, n, seed=seed))
+
+
+def small_box(cells=16, bx=5, by=9, seed=7, material=None, **kw):
+    """A small column for parity tests (seconds on the oracle)."""
+    d = dict(name=f"small_{cells}", N=cells + 1, h=1.0 / cells, dt=1e-5, steps=20,
+             material=material or REFERENCE_MATERIAL, g_mag=9.8, ramp_time=0.1,
+             **BOX_FRICTION, particles=block(bx, by, cells, seed=seed))
+    d.update(kw)
+    return d

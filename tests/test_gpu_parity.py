@@ -1,0 +1,215 @@
+"""GPU parity: libmpm_hip.so against the CPU oracle on identical inputs.
+
+The bar is bitwise equality of every particle and node field (compared as
+values, so +0 == -0).  That is stricter than the 1e-12 per-step / 1e-8
+after-100-steps tolerance the workloads are specified with; the tolerance is
+asserted as well so a failure reports both.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pympm
+import pyoracle
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+PFIELDS = ["x", "y", "vx", "vy", "v", "txx", "txy", "tyy", "tzz", "gammap", "gammadotp",
+           "ux", "uy"]
+AUX = ["exx", "exy", "eyy", "wxy", "ax", "ay", "xl", "yl"]
+NFIELDS = ["m", "mx", "my", "fx", "fy", "vx", "vy"]
+REL_TOL_1STEP = 1e-12
+REL_TOL_100 = 1e-8
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_sim(w, particles=None, flags=0, cap=0, bc_kind=None):
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w.get("bc_kind", 0) if bc_kind is None else bc_kind,
+                            bc_fp64=w.get("bc_fp64", ()), bc_int=w.get("bc_int", ()),
+                            g_mag=w.get("g_mag", 9.8), ramp_time=w.get("ramp_time", 0.1),
+                            bin_capacity=cap, flags=flags)
+    return pympm.Sim(cfg, particles if particles is not None else w["particles"])
+
+
+def compare(gpu, ref, fields, what, tol=0.0):
+    bad = []
+    for k in fields:
+        a, b = gpu[k], ref[k]
+        same = (a == b) | (np.isnan(a) & np.isnan(b))
+        if not same.all():
+            scale = np.maximum(np.abs(b), np.abs(b).max() * 1e-3 + 1e-300)
+            rel = np.max(np.abs(a - b) / scale)
+            bad.append(f"{k}: {int((~same).sum())} differ, max rel {rel:.3e}")
+            if tol and rel > tol:
+                raise AssertionError(f"{what}: {k} beyond tolerance {tol}: {bad[-1]}")
+    assert not bad, f"{what}: not bitwise: " + "; ".join(bad)
+
+
+def run_both(w, steps, threads=4, particles=None, flags=pympm.STORE_AUX, cap=0):
+    parts = particles if particles is not None else w["particles"]
+    say(f"case {w['name']}: np={len(parts['x'])} N={w['N']} steps={steps}")
+    sim = make_sim(w, particles=parts, flags=flags, cap=cap)
+    o = pyoracle.from_workload(w, num_threads=threads, particles=parts)
+    if steps:
+        sim.step(steps)
+        sim.sync()
+        o.run(steps)
+    return sim, o
+
+
+# ---------------------------------------------------------------------------
+
+def test_device_arithmetic_is_ieee():
+    say("device exp/div/sqrt vs host")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-2.0 ** -8, 2.0 ** -8, 200000) * 10.0 ** rng.uniform(-8, 0, 200000),
+                        rng.uniform(-740, 709, 100000), [0.0, -0.0, 1e-320, -1e-320, 700.0]])
+    dev = pympm.eval_ops(0, x)
+    host = np.array([pympm.exp_host(v) for v in x[:20000]])
+    assert np.array_equal(dev[:20000].view(np.uint64), host.view(np.uint64))
+    a = rng.normal(size=300000) * 10.0 ** rng.uniform(-300, 300, 300000)
+    b = rng.normal(size=300000) * 10.0 ** rng.uniform(-300, 300, 300000)
+    assert np.array_equal(pympm.eval_ops(1, a, b).view(np.uint64), (a / b).view(np.uint64))
+    s = np.abs(a)
+    assert np.array_equal(pympm.eval_ops(2, s).view(np.uint64), np.sqrt(s).view(np.uint64))
+
+
+def test_one_step_bitwise_all_fields():
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    sim, o = run_both(w, 1)
+    gp = {k: sim.get(k) for k in PFIELDS + AUX}
+    rp = o.particles()
+    compare(gp, rp, PFIELDS + AUX, "particles after 1 step", tol=REL_TOL_1STEP)
+    gn = {k: sim.node(k) for k in NFIELDS}
+    compare(gn, o.nodes(), NFIELDS, "nodes after 1 step", tol=REL_TOL_1STEP)
+    assert np.array_equal(sim.active(), rp["active"])
+    t, by = sim.time()
+    assert t == o.time()
+    assert by == rp["by"][0]
+
+
+CASES = {
+    "box_reference_material": lambda: workloads.small_box(cells=16, bx=5, by=9, steps=100),
+    "box_baseline_material": lambda: workloads.small_box(cells=32, bx=9, by=20, steps=100,
+                                                         material=workloads.BASELINE_MATERIAL,
+                                                         g_mag=9.81),
+    "silo_64": lambda: dict(workloads.silo(seed=5, cells=64), steps=100),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_100_steps_bitwise(case):
+    w = CASES[case]()
+    sim, o = run_both(w, w["steps"])
+    rp = o.particles()
+    compare({k: sim.get(k) for k in PFIELDS + AUX}, rp, PFIELDS + AUX,
+            f"{case} after {w['steps']} steps", tol=REL_TOL_100)
+    compare({k: sim.node(k) for k in NFIELDS}, o.nodes(), NFIELDS, f"{case} nodes")
+    assert np.array_equal(sim.active(), rp["active"])
+
+
+def test_shuffled_particle_order():
+    """The fold order is by particle index inside each cell, whatever the
+    memory order of the particles."""
+    w = workloads.small_box(cells=16, bx=6, by=10, steps=60)
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(len(w["particles"]["x"]))
+    parts = {k: v[perm].copy() for k, v in w["particles"].items()}
+    sim, o = run_both(w, w["steps"], particles=parts)
+    compare({k: sim.get(k) for k in PFIELDS}, o.particles(), PFIELDS, "shuffled")
+
+
+def test_host_mask_equals_box_plugin():
+    """MPM_BC_MASK with the mask the box plugin generates gives the same
+    trajectory as the device-evaluated box."""
+    w = workloads.small_box(cells=16, bx=5, by=9, steps=50)
+    o = pyoracle.from_workload(w)
+    o.run(1)
+    m = o.bc_mask()
+    mask = ((m[:, 0] != 0) * 1 + (m[:, 1] != 0) * 2).astype(np.uint8)
+    say("case mask-vs-box")
+    a = make_sim(w)
+    b = make_sim(w, bc_kind=pympm.BC_MASK)
+    b.set_bc_mask(mask)
+    a.step(50)
+    b.step(50)
+    compare({k: b.get(k) for k in PFIELDS}, {k: a.get(k) for k in PFIELDS}, PFIELDS, "mask vs box")
+
+
+def test_particles_leaving_the_grid():
+    """Particles thrown out of [0,1)^2 are deactivated one step after they
+    leave (process_usl.c:660-665) and stop contributing."""
+    w = workloads.small_box(cells=16, bx=4, by=4, steps=40, bc_kind=0, g_mag=0.0)
+    parts = {k: v.copy() for k, v in w["particles"].items()}
+    parts["x"] += 0.02
+    parts["y"] += 0.02
+    parts["vx"][:] = -2.0
+    parts["vy"][::3] = -1.0
+    sim, o = run_both(w, w["steps"], particles=parts)
+    rp = o.particles()
+    assert 0 < rp["active"].sum() < len(rp["active"])
+    assert np.array_equal(sim.active(), rp["active"])
+    compare({k: sim.get(k) for k in PFIELDS}, rp, PFIELDS, "leaving the grid")
+
+
+def test_bin_overflow_is_reported():
+    """More particles in one cell than the bin capacity: the step reports
+    MPM_HIP_ERR_BIN_OVERFLOW instead of returning wrong sums."""
+    w = workloads.small_box(cells=16, bx=1, by=1, steps=1)
+    parts = workloads.block(1, 1, 16, ppc_side=5, seed=3, cell_x0=3, cell_y0=3)   # 25 in one cell
+    say("case overflow cap=16")
+    with pytest.raises(pympm.MpmHipError) as e:
+        sim = make_sim(w, particles=parts, cap=16)
+        sim.step(1)
+        sim.sync()
+    assert e.value.code == -4
+    sim = make_sim(w, particles=parts, cap=32)
+    sim.step(2)
+    sim.sync()
+
+
+def test_column_collapse_1000_steps():
+    """The 64 x 64 column collapse (8,192 particles), all 1000 steps."""
+    w = workloads.column_collapse()
+    sim, o = run_both(w, w["steps"], threads=8, flags=0)
+    compare({k: sim.get(k) for k in PFIELDS}, o.particles(), PFIELDS, "column collapse",
+            tol=REL_TOL_100)
+
+
+def test_full_size_silo_five_steps():
+    """The 512 x 512 silo with 1,048,576 particles at full size: five steps
+    bitwise against the oracle on all host threads."""
+    w = workloads.silo()
+    threads = min(16, os.cpu_count() or 1)
+    sim, o = run_both(w, 5, threads=threads, flags=0)
+    compare({k: sim.get(k) for k in PFIELDS}, o.particles(), PFIELDS, "silo 1M, 5 steps")
+    compare({k: sim.node(k) for k in NFIELDS}, o.nodes(), NFIELDS, "silo 1M nodes")
+
+
+def test_full_size_silo_properties_after_200_steps():
+    """Size-independent properties at full size over more steps: mass is
+    conserved, a rerun is bitwise identical, nothing leaves the silo yet."""
+    w = workloads.silo()
+    say("case silo 1M x 200 steps twice")
+    runs = []
+    for _ in range(2):
+        sim = make_sim(w)
+        sim.step(200)
+        sim.sync()
+        runs.append({k: sim.get(k) for k in ("x", "y", "vy", "txx", "v")})
+        act = sim.active()
+        nm = sim.node("m")
+        sim.close()
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k].view(np.uint64), runs[1][k].view(np.uint64)), k
+    assert act.sum() == len(act)
+    assert nm.sum() == pytest.approx(w["particles"]["m"].sum(), rel=1e-12)
+    assert np.all(runs[0]["vy"] <= 1e-9)

@@ -1,0 +1,508 @@
+"""Pins the CPU oracle (oracle/usl_oracle.c) before it is trusted as the
+parity checker.
+
+Each check is chosen so that a plausible restatement mistake (a dropped term,
+a wrong sign or index, a transposed operand, a wrong fold order) fails it:
+
+* bitwise agreement with the reference's own libmpm/{interpolate,particle,
+  element,tensor}.c, compiled from source by oracle/Makefile (the rest of
+  libmpm needs <suitesparse/cs.h>, absent here);
+* closed forms of the bilinear tent functions;
+* the P2G scatter against exact rational arithmetic on dyadic inputs (every
+  product and sum is then exact, so the floating-point result must equal
+  the textbook MPM sums f_i = sum_p [N_i m_p b - V_p sigma_p . grad N_i]);
+* free fall under the reference's gravity ramp and rigid translation, whose
+  trajectories have closed forms;
+* the g_local_mu2 return map against the mu(I) rheology it discretises
+  (tau / p = mu_s + (mu_2 - mu_s) I / (I_0 + I), I = gdot d / sqrt(p/rho_s));
+* negative_root against an arbitrary-precision quadratic root;
+* thread-count independence and determinism of the whole step.
+"""
+import ctypes
+import math
+import os
+import random
+import subprocess
+from fractions import Fraction
+
+import mpmath
+import numpy as np
+import pytest
+
+import pyoracle
+import workloads
+
+D = ctypes.c_double
+PD = ctypes.POINTER(ctypes.c_double)
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ref():
+    if not os.path.exists(pyoracle.REF_PIECES):
+        pytest.skip("oracle/_ref not built: the reference tree is not mounted here")
+    lib = ctypes.CDLL(pyoracle.REF_PIECES)
+    lib.global_to_local_coords.argtypes = [PD, PD] + [D] * 5
+    lib.tent.argtypes = [PD] * 4 + [D] * 2
+    lib.grad_tent.argtypes = [PD] * 8 + [D] * 3
+    for name in ("tensor_multiply3", "tensor_add3"):
+        getattr(lib, name).argtypes = [PD, PD, PD]
+    for name in ("tensor_sym3", "tensor_skw3", "tensor_copy3"):
+        getattr(lib, name).argtypes = [PD, PD]
+    lib.tensor_decompose3.argtypes = [PD, PD, PD]
+    lib.tensor_contraction3.argtypes = [PD, PD, PD]
+    lib.tensor_trace3.argtypes = [PD, PD]
+    lib.tensor_scale3.argtypes = [PD, D]
+    return lib
+
+
+def _bits(a):
+    return np.asarray(a, dtype=np.float64).view(np.uint64)
+
+
+def _dd(n):
+    return (D * n)()
+
+
+# ---------------------------------------------------------------------------
+# Reference pieces, bitwise
+# ---------------------------------------------------------------------------
+
+def test_local_coords_and_tent_match_reference_bitwise():
+    ref = _ref()
+    lib = pyoracle.load()
+    for fn in ("oracle_tensor_multiply3", "oracle_tensor_add3", "oracle_tensor_decompose3",
+               "oracle_tensor_contraction3"):
+        getattr(lib, fn).argtypes = [PD, PD, PD]
+    rng = random.Random(11)
+    for trial in range(20000):
+        h = rng.choice([1 / 64, 1 / 512, 1 / 100, 0.013, 1 / 3])
+        xr, yr = rng.randrange(0, 50) * h, rng.randrange(0, 50) * h
+        kind = trial % 4
+        if kind == 0:
+            lx, ly = rng.random(), rng.random()
+        elif kind == 1:   # just beyond +-1: the snapping window
+            lx = rng.choice([1, -1]) * (1 + rng.random() * 2e-10)
+            ly = rng.choice([1, -1]) * (1 + rng.random() * 2e-10)
+        elif kind == 2:
+            lx, ly = rng.choice([0.0, 1.0, 0.5]), rng.choice([0.0, 1.0, 0.5])
+        else:
+            lx, ly = rng.uniform(-1.5, 2.5), rng.uniform(-1.5, 2.5)
+        x, y = xr + lx * h, yr + ly * h
+        a, b, c, d = D(), D(), D(), D()
+        ref.global_to_local_coords(ctypes.byref(a), ctypes.byref(b), x, y, xr, yr, h)
+        lib.oracle_global_to_local_coords(ctypes.byref(c), ctypes.byref(d), x, y, xr, yr, h)
+        assert _bits([a.value, b.value]).tolist() == _bits([c.value, d.value]).tolist()
+        xl, yl = a.value, b.value
+        r4, o4 = [D() for _ in range(4)], [D() for _ in range(4)]
+        ref.tent(*[ctypes.byref(q) for q in r4], xl, yl)
+        lib.oracle_tent(*[ctypes.byref(q) for q in o4], xl, yl)
+        assert _bits([q.value for q in r4]).tolist() == _bits([q.value for q in o4]).tolist()
+        r8, o8 = [D() for _ in range(8)], [D() for _ in range(8)]
+        ref.grad_tent(*[ctypes.byref(q) for q in r8], xl, yl, h)
+        lib.oracle_grad_tent(*[ctypes.byref(q) for q in o8], xl, yl, h)
+        assert _bits([q.value for q in r8]).tolist() == _bits([q.value for q in o8]).tolist()
+
+
+def test_tensor_helpers_match_reference_bitwise():
+    ref = _ref()
+    lib = pyoracle.load()
+    for fn in ("oracle_tensor_multiply3", "oracle_tensor_add3", "oracle_tensor_decompose3",
+               "oracle_tensor_contraction3"):
+        getattr(lib, fn).argtypes = [PD, PD, PD]
+    for fn in ("oracle_tensor_sym3", "oracle_tensor_skw3", "oracle_tensor_trace3"):
+        getattr(lib, fn).argtypes = [PD, PD]
+    lib.oracle_tensor_scale3.argtypes = [PD, D]
+    rng = np.random.default_rng(5)
+    for _ in range(3000):
+        A = (D * 9)(*rng.normal(size=9) * 10.0 ** rng.integers(-8, 8))
+        B = (D * 9)(*rng.normal(size=9) * 10.0 ** rng.integers(-8, 8))
+        for rfn, ofn in (("tensor_multiply3", "oracle_tensor_multiply3"),):
+            C1, C2 = _dd(9), _dd(9)
+            getattr(ref, rfn)(C1, A, B)
+            getattr(lib, ofn)(C2, A, B)
+            assert _bits(list(C1)).tolist() == _bits(list(C2)).tolist()
+        for rfn, ofn in (("tensor_sym3", "oracle_tensor_sym3"), ("tensor_skw3", "oracle_tensor_skw3")):
+            C1, C2 = _dd(9), _dd(9)
+            getattr(ref, rfn)(C1, A)
+            getattr(lib, ofn)(C2, A)
+            assert _bits(list(C1)).tolist() == _bits(list(C2)).tolist()
+        # in place C == A, as g_local_mu2 calls it
+        C1, C2 = (D * 9)(*A), (D * 9)(*A)
+        ref.tensor_add3(C1, C1, B)
+        lib.oracle_tensor_add3(C2, C2, B)
+        assert _bits(list(C1)).tolist() == _bits(list(C2)).tolist()
+        C1, C2, c1, c2 = _dd(9), _dd(9), _dd(1), _dd(1)
+        ref.tensor_decompose3(C1, c1, A)
+        lib.oracle_tensor_decompose3(C2, c2, A)
+        assert _bits(list(C1) + [c1[0]]).tolist() == _bits(list(C2) + [c2[0]]).tolist()
+        ref.tensor_contraction3(c1, A, B)
+        lib.oracle_tensor_contraction3(c2, A, B)
+        assert _bits([c1[0]]).tolist() == _bits([c2[0]]).tolist()
+        ref.tensor_trace3(c1, A)
+        lib.oracle_tensor_trace3(c2, A)
+        assert _bits([c1[0]]).tolist() == _bits([c2[0]]).tolist()
+        s = float(rng.normal())
+        C1, C2 = (D * 9)(*A), (D * 9)(*A)
+        ref.tensor_scale3(C1, s)
+        lib.oracle_tensor_scale3(C2, s)
+        assert _bits(list(C1)).tolist() == _bits(list(C2)).tolist()
+
+
+def test_abi_layout_matches_reference_headers(tmp_path):
+    """include/mpm_abi.h reproduces every offset of the reference structs
+    (golden: tests/golden/ref_layout.txt, probed from the reference headers
+    by oracle/ref_layout_probe.c)."""
+    golden = open(os.path.join(REPO, "tests", "golden", "ref_layout.txt")).read().split("\n")
+    lines = [ln for ln in golden if ln.strip()]
+    probe = tmp_path / "probe.c"
+    body = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_abi.h"',
+            'int main(void){']
+    for ln in lines:
+        t, f, _ = ln.split()
+        if f == "sizeof":
+            body.append(f'printf("{t} sizeof %zu\\n", sizeof({t}));')
+        else:
+            body.append(f'printf("{t} {f} %zu\\n", offsetof({t}, {f}));')
+    body.append("return 0;}")
+    probe.write_text("\n".join(body))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=gnu99", "-I", os.path.join(REPO, "include"), str(probe),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert [ln for ln in out.split("\n") if ln.strip()] == lines
+
+
+# ---------------------------------------------------------------------------
+# Closed forms of the shape functions
+# ---------------------------------------------------------------------------
+
+def _tent(lib, xl, yl, h):
+    s = [D() for _ in range(4)]
+    b = [D() for _ in range(8)]
+    lib.oracle_tent(*[ctypes.byref(q) for q in s], xl, yl)
+    lib.oracle_grad_tent(*[ctypes.byref(q) for q in b], xl, yl, h)
+    return [q.value for q in s], [q.value for q in b]
+
+
+def test_tent_closed_forms():
+    lib = pyoracle.load()
+    h = 1 / 64
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]   # BL BR TR TL
+    for k, (cx, cy) in enumerate(corners):
+        s, _ = _tent(lib, float(cx), float(cy), h)
+        assert s == [1.0 if j == k else 0.0 for j in range(4)]
+    s, b = _tent(lib, 0.5, 0.5, h)
+    assert s == [0.25] * 4
+    assert b == [-32.0, 32.0, 32.0, -32.0, -32.0, -32.0, 32.0, 32.0]
+    rng = np.random.default_rng(3)
+    for xl, yl in rng.random((2000, 2)):
+        s, b = _tent(lib, xl, yl, h)
+        assert abs(sum(s) - 1.0) <= 4e-16
+        assert sum(b[:4]) == 0.0 and sum(b[4:]) == 0.0
+        # gradient = analytic derivative of the bilinear interpolant
+        assert b[0] == pytest.approx(-(1 - yl) / h, rel=1e-15)
+        assert b[6] == pytest.approx(xl / h, rel=1e-15)
+
+
+def test_local_coordinate_snapping():
+    lib = pyoracle.load()
+    a, b = D(), D()
+    h = 1 / 64
+    lib.oracle_global_to_local_coords(ctypes.byref(a), ctypes.byref(b),
+                                      h * (1 + 5e-11), -h * (1 + 5e-11), 0.0, 0.0, h)
+    assert (a.value, b.value) == (1.0, -1.0)
+    lib.oracle_global_to_local_coords(ctypes.byref(a), ctypes.byref(b),
+                                      h * (1 + 1e-8), h * 0.25, 0.0, 0.0, h)
+    assert a.value > 1.0 and b.value == 0.25
+
+
+# ---------------------------------------------------------------------------
+# Element map and masks
+# ---------------------------------------------------------------------------
+
+def _one_particle_set(xs, ys):
+    n = len(xs)
+    z = np.zeros(n)
+    return {"m": np.full(n, 1e-3), "v": np.full(n, 1e-6), "x": np.asarray(xs, float),
+            "y": np.asarray(ys, float), "vx": z, "vy": z, "sxx": z, "sxy": z, "syy": z}
+
+
+def test_element_assignment_and_deactivation():
+    lib = pyoracle.load()
+    N, h = 9, 1 / 8
+    q = pyoracle.make_params(lib, N, h, 1e-5)
+    pts = _one_particle_set([0.01, 0.99, 0.5, 1.0, -1e-300, 0.3, 0.999999],
+                            [0.01, 0.99, 0.5, 0.2, 0.5, 1.0, 0.0])
+    o = pyoracle.Oracle(q, pts)
+    p = o.particles()
+    assert p["in_element"][:3].tolist() == [0, 7 + 7 * 8, 4 + 4 * 8]
+    assert p["active"].tolist() == [1, 1, 1, 0, 0, 0, 1]
+    assert p["in_element"][6] == 7
+
+
+def test_box_and_silo_masks():
+    lib = pyoracle.load()
+    N = 17
+    pts = _one_particle_set([0.3], [0.3])
+    q = pyoracle.make_params(lib, N, 1 / 16, 1e-5, bc_kind=1, bc_fp64=(0, 0, 1, 1),
+                             bc_int=(1, 3, 1, 0))
+    o = pyoracle.Oracle(q, pts)
+    o.run(1)
+    mask = o.bc_mask()
+    fx = np.zeros((N, N), bool)
+    fy = np.zeros((N, N), bool)
+    fx[:, 0] = fx[:, N - 1] = fx[0, :] = True   # [row, col]
+    fy[0, :] = True
+    assert np.array_equal(mask[:, 0].reshape(N, N) != 0, fx)
+    assert np.array_equal(mask[:, 1].reshape(N, N) != 0, fy)
+
+    q = pyoracle.make_params(lib, N, 1 / 16, 1e-5, bc_kind=2, bc_fp64=(0.05, 0.0, 0.6))
+    o = pyoracle.Oracle(q, pts)
+    o.run(1)
+    mask = o.bc_mask()
+    wall = int(16 * 0.6)
+    fx = np.zeros((N, N), bool)
+    fy = np.zeros((N, N), bool)
+    fx[:, 0] = fx[:, wall] = True
+    fx[0, 1:] = fy[0, 1:] = True     # floor, minus the open orifice at x <= 0.05
+    assert np.array_equal(mask[:, 0].reshape(N, N) != 0, fx)
+    assert np.array_equal(mask[:, 1].reshape(N, N) != 0, fy)
+
+
+# ---------------------------------------------------------------------------
+# P2G against exact rational arithmetic
+# ---------------------------------------------------------------------------
+
+def _exact_p2g(pts, N, h, dt):
+    """Textbook MPM sums in exact rationals: m_i = sum N_i m_p,
+    p_i = sum N_i m_p v_p, f_i = sum (N_i m_p b - V_p sigma_p grad N_i),
+    then p_i += dt f_i.  Body force is zero in the first step."""
+    F = Fraction
+    hh = F(h)
+    nn = N * N
+    m = [F(0)] * nn
+    px, py, fx, fy = ([F(0)] * nn for _ in range(4))
+    for k in range(len(pts["x"])):
+        x, y = F(pts["x"][k]), F(pts["y"][k])
+        c, r = int(x / hh), int(y / hh)
+        xi, eta = x / hh - c, y / hh - r
+        mp, vp = F(pts["m"][k]), F(pts["v"][k])
+        ux, uy = F(pts["vx"][k]), F(pts["vy"][k])
+        sxx, sxy, syy = F(pts["sxx"][k]), F(pts["sxy"][k]), F(pts["syy"][k])
+        for (dx, dy) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+            nx = xi if dx else 1 - xi
+            ny = eta if dy else 1 - eta
+            Ni = nx * ny
+            # d/dx of (xi or 1-xi)(eta or 1-eta)
+            gx = (1 if dx else -1) * ny / hh
+            gy = (1 if dy else -1) * nx / hh
+            i = (r + dy) * N + (c + dx)
+            m[i] += Ni * mp
+            px[i] += Ni * mp * ux
+            py[i] += Ni * mp * uy
+            fx[i] += -vp * (sxx * gx + sxy * gy)
+            fy[i] += -vp * (sxy * gx + syy * gy)
+    dtf = F(dt)
+    px = [a + dtf * b if mm > F(1, 10 ** 10) else a for a, b, mm in zip(px, fx, m)]
+    py = [a + dtf * b if mm > F(1, 10 ** 10) else a for a, b, mm in zip(py, fy, m)]
+    return m, px, py, fx, fy
+
+
+def test_p2g_matches_exact_rational_sums():
+    lib = pyoracle.load()
+    N, h, dt = 5, 0.25, 2.0 ** -12
+    rng = random.Random(17)
+    n = 40
+    q64 = lambda lo, hi: rng.randrange(lo, hi) / 64.0    # dyadic, few bits
+    pts = {"x": np.array([q64(1, 63) for _ in range(n)]), "y": np.array([q64(1, 63) for _ in range(n)]),
+           "m": np.array([q64(1, 64) for _ in range(n)]), "v": np.array([q64(1, 64) for _ in range(n)]),
+           "vx": np.array([q64(-64, 64) for _ in range(n)]), "vy": np.array([q64(-64, 64) for _ in range(n)]),
+           "sxx": np.array([q64(-64, 64) for _ in range(n)]), "sxy": np.array([q64(-64, 64) for _ in range(n)]),
+           "syy": np.array([q64(-64, 64) for _ in range(n)])}
+    q = pyoracle.make_params(lib, N, h, dt, material=dict(E=1.0, nu=0.0))
+    o = pyoracle.Oracle(q, pts)
+    o.run(1)
+    nd = o.nodes()
+    m, px, py, fx, fy = _exact_p2g(pts, N, h, dt)
+    assert [Fraction(v) for v in nd["m"]] == m
+    assert [Fraction(v) for v in nd["fx"]] == fx
+    assert [Fraction(v) for v in nd["fy"]] == fy
+    assert [Fraction(v) for v in nd["mx"]] == px
+    assert [Fraction(v) for v in nd["my"]] == py
+
+
+# ---------------------------------------------------------------------------
+# Trajectories with closed forms
+# ---------------------------------------------------------------------------
+
+def test_free_fall_follows_gravity_ramp():
+    """No walls, no stress: every particle falls with the reference's gravity
+    (loading.c: -g * t / 0.1 until t = 0.1, applied after each step)."""
+    lib = pyoracle.load()
+    pts = workloads.block(6, 6, 32, seed=4, cell_x0=13, cell_y0=20)
+    dt, g, ramp, nsteps = 1e-4, 9.8, 0.1, 150
+    q = pyoracle.make_params(lib, 33, 1 / 32, dt, material=dict(E=1e3, nu=0.3),
+                             g_mag=g, ramp_time=ramp)
+    o = pyoracle.Oracle(q, pts)
+    o.run(nsteps)
+    p = o.particles()
+    v, y = 0.0, 0.0
+    for k in range(1, nsteps + 1):
+        t_prev = (k - 1) * dt
+        b = 0.0 if k == 1 else (-g * t_prev / ramp if t_prev < ramp else -g)
+        v += dt * b
+        y += dt * v
+    assert np.allclose(p["vy"], v, rtol=1e-11, atol=0)
+    assert np.allclose(p["y"] - pts["y"], y, rtol=1e-9, atol=0)
+    assert np.allclose(p["vx"], 0.0, atol=1e-14)
+    assert o.time() == pytest.approx(nsteps * dt, rel=1e-13)
+
+
+def test_rigid_translation():
+    lib = pyoracle.load()
+    pts = workloads.block(5, 4, 32, seed=9, cell_x0=10, cell_y0=10)
+    pts["vx"][:] = 0.125
+    pts["vy"][:] = -0.0625
+    dt, n = 1e-4, 20
+    q = pyoracle.make_params(lib, 33, 1 / 32, dt, material=workloads.REFERENCE_MATERIAL, g_mag=0.0)
+    o = pyoracle.Oracle(q, pts)
+    o.run(n)
+    p = o.particles()
+    assert np.allclose(p["x"], pts["x"] + n * dt * 0.125, rtol=0, atol=1e-15)
+    assert np.allclose(p["y"], pts["y"] - n * dt * 0.0625, rtol=0, atol=1e-15)
+    for k in ("exx", "eyy", "exy", "wxy"):
+        assert np.abs(p[k]).max() < 1e-12
+    assert np.allclose(p["v"], pts["v"], rtol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# g_local_mu2 against the mu(I) rheology
+# ---------------------------------------------------------------------------
+
+def _stress(lib, T, m, v, gp, L, dt, q):
+    out = _dd(6)
+    Tin = (D * 4)(*T)
+    lib.oracle_stress_point(Tin, m, v, gp, L[0], L[1], L[2], L[3], dt, ctypes.byref(q), out)
+    return list(out)
+
+
+def test_g_local_mu2_follows_mu_of_I():
+    lib = pyoracle.load()
+    q = pyoracle.make_params(lib, 9, 0.125, 1e-5, material=workloads.BASELINE_MATERIAL)
+    mat = workloads.BASELINE_MATERIAL
+    E, nu = mat["E"], mat["nu"]
+    G = E / (2 * (1 + nu))
+    rng = np.random.default_rng(8)
+    n_plastic = 0
+    for _ in range(3000):
+        p0 = 10 ** rng.uniform(1, 4)
+        T = [-p0 + rng.normal() * p0 * 0.3, rng.normal() * p0 * 0.4,
+             -p0 + rng.normal() * p0 * 0.3, -p0 + rng.normal() * p0 * 0.1]
+        L = rng.normal(size=4) * 10 ** rng.uniform(-1, 2)
+        dt = 10 ** rng.uniform(-6, -4)
+        out = _stress(lib, T, 1500.0, 1.0, 0.0, L, dt, q)
+        txx, txy, tyy, tzz, gp, gdp = out
+        p = -(txx + tyy + tzz) / 3
+        dev = [txx + p, txy, tyy + p, tzz + p]
+        tau = math.sqrt(0.5 * (dev[0] ** 2 + 2 * dev[1] ** 2 + dev[2] ** 2 + dev[3] ** 2))
+        assert gdp >= 0.0
+        assert gp == pytest.approx(gdp * dt, rel=1e-15)
+        if p <= 0:
+            assert out[:4] == [0.0, 0.0, 0.0, 0.0] and gdp == 0.0
+            continue
+        if gdp > 0:
+            n_plastic += 1
+            inertial = gdp * mat["grains_d"] / math.sqrt(p / mat["grains_rho"])
+            mu = mat["mu_s"] + (mat["mu_2"] - mat["mu_s"]) * inertial / (mat["I_0"] + inertial)
+            assert tau / p == pytest.approx(mu, rel=1e-9, abs=1e-12)
+            assert mat["mu_s"] * p * (1 - 1e-12) <= tau <= mat["mu_2"] * p * (1 + 1e-12)
+        else:
+            assert tau <= mat["mu_s"] * p * (1 + 1e-12)
+    assert n_plastic > 300
+
+
+def test_g_local_mu2_cutoffs_and_rotation():
+    lib = pyoracle.load()
+    q = pyoracle.make_params(lib, 9, 0.125, 1e-5, material=workloads.REFERENCE_MATERIAL)
+    # below the critical density: stress dropped
+    out = _stress(lib, [-5.0, 1.0, -4.0, -3.0], 1484.0, 1.0, 0.0, [0.1, 0.2, 0.3, 0.4], 1e-5, q)
+    assert out[:4] == [0.0, 0.0, 0.0, 0.0] and out[5] == 0.0
+    # tension: stress dropped
+    out = _stress(lib, [5.0, 0.0, 5.0, 5.0], 2000.0, 1.0, 0.0, [0.0, 0.0, 0.0, 0.0], 1e-5, q)
+    assert out[:4] == [0.0, 0.0, 0.0, 0.0]
+    # pure spin of an isotropic state: W T - T W = 0, the state is unchanged
+    out = _stress(lib, [-7.0, 0.0, -7.0, -7.0], 2000.0, 1.0, 0.0, [0.0, 0.0, 0.0, 3.0], 1e-4, q)
+    assert out[:4] == [-7.0, 0.0, -7.0, -7.0] and out[5] == 0.0
+
+
+def test_negative_root_against_arbitrary_precision():
+    lib = pyoracle.load()
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(12)
+    for _ in range(2000):
+        s2, tau, alpha, s0 = (10 ** rng.uniform(-2, 6, size=4))
+        s0 = min(s0, s2)
+        B = -(s2 + tau + alpha)
+        H = s2 * tau + s0 * alpha
+        got = lib.oracle_negative_root(1.0, B, H)
+        Bm, Hm = mpmath.mpf(B), mpmath.mpf(H)
+        exact = (-Bm - mpmath.sqrt(Bm * Bm - 4 * Hm)) / 2
+        assert got == pytest.approx(float(exact), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# exp
+# ---------------------------------------------------------------------------
+
+def test_portable_exp_against_glibc_and_correct_rounding():
+    lib = pyoracle.load("parity")
+    mpmath.mp.prec = 200
+    rng = np.random.default_rng(21)
+    small = rng.uniform(-2.0 ** -8, 2.0 ** -8, 20000) * 10.0 ** rng.uniform(-6, 0, 20000)
+    for x in small:
+        got = lib.oracle_exp(float(x))
+        cr = float(mpmath.exp(mpmath.mpf(float(x))))
+        assert got == cr
+    for x in rng.uniform(-700, 700, 20000):
+        got = lib.oracle_exp(float(x))
+        ref = math.exp(float(x))
+        ulp = math.ulp(ref)
+        assert abs(got - ref) <= ulp
+    assert lib.oracle_exp(0.0) == 1.0
+    assert lib.oracle_exp(-1000.0) == 0.0
+    assert math.isinf(lib.oracle_exp(1000.0))
+    assert math.isnan(lib.oracle_exp(float("nan")))
+
+
+# ---------------------------------------------------------------------------
+# Whole-step properties
+# ---------------------------------------------------------------------------
+
+def test_thread_count_independence_and_determinism():
+    w = workloads.small_box(cells=16, bx=7, by=11, steps=40)
+    runs = []
+    for T in (1, 2, 4, 7, 1):
+        o = pyoracle.from_workload(w, num_threads=T)
+        o.run(w["steps"])
+        runs.append(o.particles())
+    for r in runs[1:]:
+        for k in ("x", "y", "vx", "vy", "v", "txx", "txy", "tyy", "tzz", "gammap"):
+            assert np.array_equal(_bits(runs[0][k]), _bits(r[k])), k
+
+
+def test_mass_and_momentum_bookkeeping():
+    """Node mass equals particle mass; with no walls and no body force the
+    grid forces (internal stresses) sum to zero."""
+    w = workloads.small_box(cells=16, bx=6, by=6, steps=1, g_mag=0.0, bc_kind=0)
+    w["particles"]["sxx"][:] = -3.0
+    w["particles"]["syy"][:] = -2.0
+    w["particles"]["sxy"][:] = 0.5
+    w["particles"]["x"] += 4 / 16
+    w["particles"]["y"] += 4 / 16
+    o = pyoracle.from_workload(w)
+    o.run(1)
+    nd = o.nodes()
+    p = w["particles"]
+    assert nd["m"].sum() == pytest.approx(p["m"].sum(), rel=1e-14)
+    scale = np.abs(nd["fx"]).max()
+    assert abs(nd["fx"].sum()) < 1e-12 * scale and abs(nd["fy"].sum()) < 1e-12 * scale
