@@ -83,12 +83,18 @@ typedef struct mpm_hip_material {
                                   acceleration (x_tt, y_tt) and local
                                   coordinates (xl, yl); off by default, they
                                   are not read by the next step */
+#define MPM_HIP_HOST_MATERIAL 0x2  /* the step stops after G2P, strain rate
+                                  and volume update (implies STORE_AUX); the
+                                  caller runs a host material plugin and
+                                  writes TXX/TXY/TYY (the plugin's sxx, sxy,
+                                  syy) back before the next step */
 
 typedef struct mpm_hip_config {
     int N;                  /* nodes per side; grid covers [0,1]^2 */
     double h;               /* node spacing, len / (N - 1) */
     double dt;
     double t0;              /* job->t when the loop starts */
+    double loads_t0;        /* job->step_start_time (loading.c ramp origin) */
     double g_mag;           /* loading.c G_MAG, reference 9.8 */
     double ramp_time;       /* loading.c RAMP_TIME, reference 0.1 */
     mpm_hip_material material;
@@ -142,14 +148,23 @@ int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
  * Writing X or Y re-bins the particles. */
 int mpm_hip_set_field(mpm_hip_ctx *ctx, int field, const double *host);
 int mpm_hip_get_field(mpm_hip_ctx *ctx, int field, double *host);
-/* job->active and job->in_element as the reference would hold them after
- * the last completed step. */
+/* job->active as the reference holds it after the last completed step
+ * (a particle that left the grid is deactivated at the start of the step
+ * after it left, libmpm/process_usl.c:660-665). */
 int mpm_hip_get_active(mpm_hip_ctx *ctx, int *host);
 int mpm_hip_get_node_field(mpm_hip_ctx *ctx, int field, double *host);
 
 /* MPM_BC_MASK: per node, bit 0 = X DOF fixed (zero x momentum and force),
  * bit 1 = Y DOF fixed.  Takes effect from the next step. */
 int mpm_hip_set_bc_mask(mpm_hip_ctx *ctx, const uint8_t *mask);
+
+/* Body force per unit mass of the next step (what time_varying_loads or
+ * initial_loads last wrote into every particle's bx, by); afterwards the
+ * device keeps applying loading.c's rule itself. */
+int mpm_hip_set_body_force(mpm_hip_ctx *ctx, double bx, double by);
+/* Turns MPM_HIP_STORE_AUX on or off from the next step on (allocates the
+ * auxiliary arrays on first use). */
+int mpm_hip_set_store_aux(mpm_hip_ctx *ctx, int on);
 
 /* Enqueues n USL steps on the context's stream; returns without waiting. */
 int mpm_hip_step(mpm_hip_ctx *ctx, int n);

@@ -3,9 +3,12 @@
 //
 // One step is two kernels on one stream:
 //
-//   k_p2g_grid   one thread per grid node, 16x16 nodes per workgroup.  The
-//                workgroup stages the particle lists of the 17x17 cells
-//                around its tile in LDS and sorts each list by particle index.
+//   k_p2g_grid   one thread per grid node, 16x8 nodes per workgroup.  For
+//                each colour in turn the workgroup loads the particle lists of
+//                that colour's cells among the 17x9 cells around its tile,
+//                sorts each list by particle index, and stages the particles'
+//                local coordinates and mass/momentum/stress products in LDS
+//                (coalesced loads, each particle read once per workgroup).
 //                Each thread then folds, for its own node, the contributions
 //                of the four adjacent cells in colour order 0..3 and, inside
 //                a cell, of its particles in ascending index -- the exact
@@ -43,12 +46,18 @@
 namespace {
 
 constexpr double kTol = 1e-10;     // TOL in process_usl.c:25 and particle.c:10
-constexpr int kTile = 16;          // nodes per tile side in k_p2g_grid
-constexpr int kRegion = kTile + 1; // cells per staged region side
-constexpr int kP2GThreads = kTile * kTile;
+constexpr int kTileX = 16;         // nodes per tile in k_p2g_grid: 16 x 8
+constexpr int kTileY = 8;
+constexpr int kRegX = kTileX + 1;  // cells whose particles reach the tile: 17 x 9
+constexpr int kRegY = kTileY + 1;
+constexpr int kRegCells = kRegX * kRegY;
+constexpr int kColourCellsMax = ((kRegX + 1) / 2) * ((kRegY + 1) / 2);   // 9 x 5
+constexpr int kStageMax = 720;     // staged particles per batch (64 B each)
+constexpr int kP2GThreads = kTileX * kTileY;
 constexpr int kG2PThreads = 256;
 
 enum : int { kStatusBinOverflow = 1 };
+enum : int { kModeStoreAux = 1, kModeHostMaterial = 2 };
 
 struct DevState {
     double t;           // job->t
@@ -267,6 +276,36 @@ __device__ __forceinline__ void g_local_mu2_update(double exx, double exy, doubl
     gdp = nup_tau;
 }
 
+// Appends particle p to the bin of cell `key` when `valid`.  Lanes of a wave
+// that share a cell take their slots with one atomic per distinct cell (the
+// slot order inside a bin is irrelevant: readers sort the bin by index).
+// Must be reached by every lane of the wave that has not exited.
+__device__ __forceinline__ void bin_insert(bool valid, int key, int p, int *cnt, int *bkt,
+                                           int cap, int *status)
+{
+    unsigned long long pending = __ballot(valid);
+    const unsigned long long lane_bit = 1ull << __lane_id();
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int lkey = __shfl(key, leader);
+        const unsigned long long group = __ballot(valid && key == lkey) & pending;
+        int base = 0;
+        if (lane_bit == (1ull << leader)) {
+            base = atomicAdd(&cnt[lkey], __popcll(group));
+        }
+        base = __shfl(base, leader);
+        if (group & lane_bit) {
+            const int r = base + __popcll(group & (lane_bit - 1));
+            if (r < cap) {
+                bkt[(size_t)lkey * cap + r] = p;
+            } else {
+                atomicOr(status, kStatusBinOverflow);
+            }
+        }
+        pending &= ~group;
+    }
+}
+
 __device__ __forceinline__ double gravity(double t, double t0, double g_mag, double ramp)
 {
     // time_varying_loads (loading.c:34-54)
@@ -288,10 +327,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     DevState *__restrict__ st, int np, int init)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) {
-        return;
-    }
-    if (!init && !P.active[p]) {
+    if (p >= np || (!init && !P.active[p])) {
         return;
     }
     const int e = which_element(P.x[p], P.y[p], g);
@@ -299,14 +335,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     if (init) {
         P.active[p] = e >= 0;
     }
-    if (e >= 0) {
-        const int r = atomicAdd(&cnt[e], 1);
-        if (r < g.cap) {
-            bkt[(size_t)e * g.cap + r] = p;
-        } else {
-            atomicOr(&st->status, kStatusBinOverflow);
-        }
-    }
+    bin_insert(e >= 0, e, p, cnt, bkt, g.cap, &st->status);
 }
 
 template <int CAP>
@@ -316,8 +345,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
     int tiles_x, int ntiles)
 {
-    __shared__ int s_cnt[kRegion * kRegion];
-    __shared__ int s_lst[kRegion * kRegion * CAP];
+    // bin entries of one colour's cells held per thread between colours
+    constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
+    constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;
+
+    __shared__ int s_cnt[kRegCells];
+    __shared__ int s_lst[kColourCellsMax * CAP];
+    __shared__ int s_off[kColourCellsMax + 1];
+    // staged particles of one colour, as pairs for 16-byte LDS reads
+    __shared__ double2 s_xy[kStageMax];    // local coordinates
+    __shared__ double2 s_mpx[kStageMax];   // m, vx * m
+    __shared__ double2 s_pys[kStageMax];   // vy * m, -sxx * v
+    __shared__ double2 s_ss[kStageMax];    // -sxy * v, -syy * v
 
     // Consecutive tiles go to the same XCD (workgroups are dealt to the 8
     // XCDs round-robin), so neighbouring tiles share that XCD's L2.
@@ -326,22 +365,20 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     if (tile >= ntiles) {
         return;
     }
-    const int tx0 = (tile % tiles_x) * kTile;
-    const int ty0 = (tile / tiles_x) * kTile;
-    const int tid = threadIdx.x;
+    const int tx0 = (tile % tiles_x) * kTileX;
+    const int ty0 = (tile / tiles_x) * kTileY;
     const int cx0 = tx0 - 1, cy0 = ty0 - 1;   // staged cell region origin
+    const int tid = threadIdx.x;
 
     // This tile owns cells (tx0.., ty0..): clear their next-step bin counts.
     {
-        const int cc = tx0 + (tid % kTile), rr = ty0 + (tid / kTile);
+        const int cc = tx0 + (tid % kTileX), rr = ty0 + (tid / kTileX);
         if (cc < g.Ne && rr < g.Ne) {
             cnt_next[rr * g.Ne + cc] = 0;
         }
     }
-
-    // Stage list lengths, then list entries.
-    for (int cl = tid; cl < kRegion * kRegion; cl += kP2GThreads) {
-        const int cc = cx0 + cl % kRegion, rr = cy0 + cl / kRegion;
+    for (int cl = tid; cl < kRegCells; cl += kP2GThreads) {
+        const int cc = cx0 + cl % kRegX, rr = cy0 + cl / kRegX;
         int n = 0;
         if (cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
             n = cnt_cur[rr * g.Ne + cc];
@@ -350,72 +387,177 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         s_cnt[cl] = n;
     }
     __syncthreads();
-    for (int idx = tid; idx < kRegion * kRegion * CAP; idx += kP2GThreads) {
-        const int cl = idx / CAP, k = idx % CAP;
-        if (k < s_cnt[cl]) {
-            const int cc = cx0 + cl % kRegion, rr = cy0 + cl / kRegion;
-            s_lst[idx] = bkt_cur[(size_t)(rr * g.Ne + cc) * CAP + k];
-        }
-    }
-    __syncthreads();
-    // Sort every list by particle index (insertion sort; lists arrive nearly
-    // sorted because lanes of one wave insert in order).
-    for (int cl = tid; cl < kRegion * kRegion; cl += kP2GThreads) {
-        int *a = &s_lst[cl * CAP];
-        const int n = s_cnt[cl];
-        for (int k = 1; k < n; ++k) {
-            const int key = a[k];
-            int q = k - 1;
-            while (q >= 0 && a[q] > key) {
-                a[q + 1] = a[q];
-                --q;
-            }
-            a[q + 1] = key;
-        }
-    }
-    __syncthreads();
 
-    const int i = tx0 + (tid % kTile);
-    const int j = ty0 + (tid / kTile);
-    if (i >= g.N || j >= g.N) {
-        return;
-    }
+    // Colour c's cells: region column a = 2*ax + ca holds cell column cx0 + a
+    // (cx0 is odd, so colour parity pc <-> ca = 1 - pc); likewise rows.
+    auto ncx_of = [](int c) { return (c & 1) ? (kRegX + 1) / 2 : kRegX / 2; };
+    auto ncy_of = [](int c) { return (c >> 1) ? (kRegY + 1) / 2 : kRegY / 2; };
+
+    // Issues the loads of colour c's bin entries (all slots at once).
+    int lst_reg[kListSlots];
+    auto fetch_lists = [&](int c) {
+        const int ncx = ncx_of(c), nq = ncx * ncy_of(c);
+        const int ca = (c & 1) ? 0 : 1, cb = (c >> 1) ? 0 : 1;
+#pragma unroll
+        for (int u = 0; u < kListSlots; ++u) {
+            const int idx = tid + u * kP2GThreads;
+            const int q = idx / CAP, k = idx % CAP;
+            int v = -1;
+            if (q < nq) {
+                const int a = 2 * (q % ncx) + ca, b = 2 * (q / ncx) + cb;
+                if (k < s_cnt[b * kRegX + a]) {
+                    v = bkt_cur[(size_t)((cy0 + b) * g.Ne + (cx0 + a)) * CAP + k];
+                }
+            }
+            lst_reg[u] = v;
+        }
+    };
+    fetch_lists(0);
+
+    const int i = tx0 + (tid % kTileX);
+    const int j = ty0 + (tid / kTileX);
+    const bool live = i < g.N && j < g.N;
     const double bx = st->bx, by = st->by;
     double am = 0.0, amx = 0.0, amy = 0.0, afx = 0.0, afy = 0.0;
+
+    // Colours in the reference's order; colour = 2 * (row % 2) + (col % 2).
     for (int c = 0; c < 4; ++c) {
-        // the adjacent cell of colour c = 2 * (row % 2) + (col % 2)
-        const int cc = i - ((i ^ c) & 1);
-        const int rr = j - ((j ^ (c >> 1)) & 1);
-        if (cc < 0 || rr < 0 || cc >= g.Ne || rr >= g.Ne) {
-            continue;
+        const int pc = c & 1, pr = c >> 1;
+        const int ncx = ncx_of(c), ncy = ncy_of(c);
+        const int ca = pc ? 0 : 1, cb = pr ? 0 : 1;
+        const int nq = ncx * ncy;
+#pragma unroll
+        for (int u = 0; u < kListSlots; ++u) {
+            const int idx = tid + u * kP2GThreads;
+            if (idx < kColourCellsMax * CAP && lst_reg[u] >= 0) {
+                s_lst[idx] = lst_reg[u];
+            }
         }
-        const int dx = i - cc, dy = j - rr;      // node's corner in that cell
-        const int cl = (cc - cx0) + (rr - cy0) * kRegion;
-        const int n = s_cnt[cl];
-        const double xn = (double)cc * g.h, yn = (double)rr * g.h;
-        for (int k = 0; k < n; ++k) {
-            const int p = s_lst[cl * CAP + k];
-            const double x = P.x[p], y = P.y[p];
-            const double mp = P.m[p], vp = P.v[p];
-            const double vx = P.vx[p], vy = P.vy[p];
-            const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
-            const double xl = snap_local((x - xn) / g.h);
-            const double yl = snap_local((y - yn) / g.h);
-            const double X = dx ? xl : 1.0 - xl;
-            const double Y = dy ? yl : 1.0 - yl;
-            const double s = X * Y;
-            const double b1 = (dx ? Y : -Y) * g.inv_h;
-            const double b2 = (dy ? X : -X) * g.inv_h;
-            am += s * mp;
-            amx += s * (vx * mp);
-            amy += s * (vy * mp);
-            afx += s * (bx * mp);
-            afx += b1 * (-sxx * vp);
-            afx += b2 * (-sxy * vp);
-            afy += s * (by * mp);
-            afy += b1 * (-sxy * vp);
-            afy += b2 * (-syy * vp);
+        __syncthreads();
+        if (tid < nq) {
+            // sort the bin by particle index (insertion sort; bins arrive
+            // nearly sorted because lanes of a wave insert in lane order)
+            const int a = 2 * (tid % ncx) + ca, b = 2 * (tid / ncx) + cb;
+            const int n = s_cnt[b * kRegX + a];
+            int *lst = &s_lst[tid * CAP];
+            for (int k = 1; k < n; ++k) {
+                const int key = lst[k];
+                int q = k - 1;
+                while (q >= 0 && lst[q] > key) {
+                    lst[q + 1] = lst[q];
+                    --q;
+                }
+                lst[q + 1] = key;
+            }
+        } else if (tid == kP2GThreads - 1) {
+            int off = 0;
+            for (int q = 0; q < nq; ++q) {
+                s_off[q] = off;
+                off += s_cnt[(2 * (q / ncx) + cb) * kRegX + 2 * (q % ncx) + ca];
+            }
+            s_off[nq] = off;
         }
+        __syncthreads();
+        if (c < 3) {
+            fetch_lists(c + 1);   // in flight while this colour is staged and folded
+        }
+
+        // This node's cell of colour c, as an index into the colour's cells.
+        int myq = -1, dx = 0, dy = 0;
+        if (live) {
+            const int cc = i - ((i ^ pc) & 1);
+            const int rr = j - ((j ^ pr) & 1);
+            if (cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
+                myq = ((rr - cy0) >> 1) * ncx + ((cc - cx0) >> 1);
+                dx = i - cc;
+                dy = j - rr;
+            }
+        }
+
+        // Stage the colour's particles in batches of whole cells.
+        int qb = 0;
+        while (qb < nq) {
+            const int base = s_off[qb];
+            int qe = qb + 1;
+            while (qe < nq && s_off[qe + 1] - base <= kStageMax) {
+                ++qe;
+            }
+            const int nb = s_off[qe] - base;
+            int sp[kStageSlots], sc[kStageSlots];
+#pragma unroll
+            for (int u = 0; u < kStageSlots; ++u) {
+                const int s = tid + u * kP2GThreads;
+                sp[u] = 0;
+                sc[u] = -1;
+                if (s < nb) {
+                    const int gi = base + s;
+                    int lo = qb, hi = qe - 1;      // last cell starting at or before gi
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= gi) lo = mid; else hi = mid - 1;
+                    }
+                    sp[u] = s_lst[lo * CAP + (gi - s_off[lo])];
+                    sc[u] = lo;
+                }
+            }
+            double px[kStageSlots], py[kStageSlots], pm[kStageSlots], pv[kStageSlots];
+            double pvx[kStageSlots], pvy[kStageSlots];
+            double psx[kStageSlots], psy[kStageSlots], psz[kStageSlots];
+#pragma unroll
+            for (int u = 0; u < kStageSlots; ++u) {
+                const int p = sp[u];    // 0 for idle slots: always a valid index
+                px[u] = P.x[p];
+                py[u] = P.y[p];
+                pm[u] = P.m[p];
+                pv[u] = P.v[p];
+                pvx[u] = P.vx[p];
+                pvy[u] = P.vy[p];
+                psx[u] = P.txx[p];
+                psy[u] = P.txy[p];
+                psz[u] = P.tyy[p];
+            }
+#pragma unroll
+            for (int u = 0; u < kStageSlots; ++u) {
+                const int s = tid + u * kP2GThreads;
+                if (sc[u] >= 0) {
+                    const int lo = sc[u];
+                    const int cc = cx0 + 2 * (lo % ncx) + ca, rr = cy0 + 2 * (lo / ncx) + cb;
+                    const double mp = pm[u], vp = pv[u];
+                    s_xy[s] = make_double2(snap_local((px[u] - (double)cc * g.h) / g.h),
+                                           snap_local((py[u] - (double)rr * g.h) / g.h));
+                    s_mpx[s] = make_double2(mp, pvx[u] * mp);     // pdata[0..1]
+                    s_pys[s] = make_double2(pvy[u] * mp, -psx[u] * vp);
+                    s_ss[s] = make_double2(-psy[u] * vp, -psz[u] * vp);
+                }
+            }
+            __syncthreads();
+            if (myq >= qb && myq < qe) {
+                const int k1 = s_off[myq + 1] - base;
+                for (int s = s_off[myq] - base; s < k1; ++s) {
+                    const double2 xy = s_xy[s], mpx = s_mpx[s], pys = s_pys[s], ss = s_ss[s];
+                    const double X = dx ? xy.x : 1.0 - xy.x;
+                    const double Y = dy ? xy.y : 1.0 - xy.y;
+                    const double sf = X * Y;
+                    const double b1 = (dx ? Y : -Y) * g.inv_h;
+                    const double b2 = (dy ? X : -X) * g.inv_h;
+                    const double mp = mpx.x;
+                    am += sf * mp;
+                    amx += sf * mpx.y;
+                    amy += sf * pys.x;
+                    afx += sf * (bx * mp);
+                    afx += b1 * pys.y;
+                    afx += b2 * ss.x;
+                    afy += sf * (by * mp);
+                    afy += b1 * ss.x;
+                    afy += b2 * ss.y;
+                }
+            }
+            __syncthreads();
+            qb = qe;
+        }
+    }
+    if (!live) {
+        return;
     }
 
     // bc_momentum then bc_force, then move_grid_split.
@@ -446,7 +588,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
 __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     Particles P, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
     int *__restrict__ bkt_next, GridParams g, Material mt, DevState *__restrict__ st,
-    int np, int store_aux)
+    int np, int mode)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
@@ -522,24 +664,25 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     const double mp = P.m[p];
     const double v = P.v[p] * mpm_exp(dt * (exx + eyy));
 
-    double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
-    double gp = P.gp[p], gdp;
-    g_local_mu2_update(exx, exy, eyy, wxy, mp, v, dt, mt, txx, txy, tyy, tzz, gp, gdp);
-
+    if (!(mode & kModeHostMaterial)) {
+        double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
+        double gp = P.gp[p], gdp;
+        g_local_mu2_update(exx, exy, eyy, wxy, mp, v, dt, mt, txx, txy, tyy, tzz, gp, gdp);
+        P.txx[p] = txx;
+        P.txy[p] = txy;
+        P.tyy[p] = tyy;
+        P.tzz[p] = tzz;
+        P.gp[p] = gp;
+        P.gdp[p] = gdp;
+    }
     P.x[p] = x;
     P.y[p] = y;
     P.vx[p] = vx;
     P.vy[p] = vy;
     P.v[p] = v;
-    P.txx[p] = txx;
-    P.txy[p] = txy;
-    P.tyy[p] = tyy;
-    P.tzz[p] = tzz;
-    P.gp[p] = gp;
-    P.gdp[p] = gdp;
     P.ux[p] = ux;
     P.uy[p] = uy;
-    if (store_aux) {
+    if (mode & kModeStoreAux) {
         P.exx[p] = exx;
         P.exy[p] = exy;
         P.eyy[p] = eyy;
@@ -553,17 +696,11 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     // The next step's create_particle_to_element_map, and its bin.
     const int en = which_element(x, y, g);
     P.elem[p] = en;
-    if (en != e) {
-        atomicAdd(&st->crossings, 1ull);
+    const unsigned long long crossed = __ballot(en != e);
+    if (crossed && __lane_id() == __builtin_ctzll(__ballot(1))) {
+        atomicAdd(&st->crossings, (unsigned long long)__popcll(crossed));
     }
-    if (en >= 0) {
-        const int r = atomicAdd(&cnt_next[en], 1);
-        if (r < g.cap) {
-            bkt_next[(size_t)en * g.cap + r] = p;
-        } else {
-            atomicOr(&st->status, kStatusBinOverflow);
-        }
-    }
+    bin_insert(en >= 0, en, p, cnt_next, bkt_next, g.cap, &st->status);
 }
 
 __global__ void k_eval_ops(int op, const double *a, const double *b, double *out, size_t n)
@@ -614,6 +751,49 @@ struct mpm_hip_ctx {
         }                                                                          \
     } while (0)
 
+static void bind_fields(mpm_hip_ctx *ctx)
+{
+    Particles &P = ctx->P;
+    double **f = ctx->pfield;
+    P.x = f[MPM_F_X];
+    P.y = f[MPM_F_Y];
+    P.vx = f[MPM_F_VX];
+    P.vy = f[MPM_F_VY];
+    P.m = f[MPM_F_M];
+    P.v = f[MPM_F_V];
+    P.txx = f[MPM_F_TXX];
+    P.txy = f[MPM_F_TXY];
+    P.tyy = f[MPM_F_TYY];
+    P.tzz = f[MPM_F_TZZ];
+    P.gp = f[MPM_F_GAMMAP];
+    P.gdp = f[MPM_F_GAMMADOTP];
+    P.ux = f[MPM_F_UX];
+    P.uy = f[MPM_F_UY];
+    P.exx = f[MPM_F_EXX];
+    P.exy = f[MPM_F_EXY];
+    P.eyy = f[MPM_F_EYY];
+    P.wxy = f[MPM_F_WXY];
+    P.ax = f[MPM_F_AX];
+    P.ay = f[MPM_F_AY];
+    P.xl = f[MPM_F_XL];
+    P.yl = f[MPM_F_YL];
+}
+
+// Auxiliary per-particle outputs, allocated (zeroed) on first use.
+static int alloc_aux(mpm_hip_ctx *ctx)
+{
+    const size_t bytes = (ctx->np ? ctx->np : 8) * sizeof(double);
+    for (int f = MPM_F_EXX; f < MPM_F_COUNT; ++f) {
+        if (ctx->pfield[f]) continue;
+        if (hipMalloc((void **)&ctx->pfield[f], bytes) != hipSuccess
+            || hipMemsetAsync(ctx->pfield[f], 0, bytes, ctx->stream) != hipSuccess) {
+            return MPM_HIP_ERR_ALLOC;
+        }
+    }
+    bind_fields(ctx);
+    return MPM_HIP_OK;
+}
+
 static int use_device(const mpm_hip_ctx *ctx)
 {
     HIP_TRY(hipSetDevice(ctx->device));
@@ -646,9 +826,12 @@ static int launch_g2p(mpm_hip_ctx *ctx)
 {
     const int nxt = 1 - ctx->cur;
     const size_t nb = ctx->np ? (ctx->np + kG2PThreads - 1) / kG2PThreads : 1;
+    int mode = 0;
+    if (ctx->cfg.flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) mode |= kModeStoreAux;
+    if (ctx->cfg.flags & MPM_HIP_HOST_MATERIAL) mode |= kModeHostMaterial;
     k_g2p_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
-        (int)ctx->np, (ctx->cfg.flags & MPM_HIP_STORE_AUX) ? 1 : 0);
+        (int)ctx->np, mode);
     HIP_TRY(hipGetLastError());
     ctx->cur = nxt;
     return MPM_HIP_OK;
@@ -677,6 +860,7 @@ void mpm_hip_default_config(mpm_hip_config *cfg)
     cfg->h = 1.0 / 64;
     cfg->dt = 1e-5;
     cfg->t0 = 0.0;
+    cfg->loads_t0 = 0.0;
     cfg->g_mag = 9.8;
     cfg->ramp_time = 0.1;
     cfg->material.E = 1e6;
@@ -793,8 +977,8 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     mt.grains_rho = cfg->material.grains_rho;
     mt.rho_crit = cfg->material.rho_crit;
 
-    ctx->tiles_x = (cfg->N + kTile - 1) / kTile;
-    ctx->ntiles = ctx->tiles_x * ctx->tiles_x;
+    ctx->tiles_x = (cfg->N + kTileX - 1) / kTileX;
+    ctx->ntiles = ctx->tiles_x * ((cfg->N + kTileY - 1) / kTileY);
     ctx->p2g_blocks = ((ctx->ntiles + 7) / 8) * 8;
 
 #define ALLOC_OR_FAIL(ptr, bytes)                                                   \
@@ -812,37 +996,15 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     }
     const size_t np = num_particles;
     const size_t nn = (size_t)cfg->N * cfg->N;
-    for (int f = 0; f < MPM_F_COUNT; ++f) {
-        const bool aux = f >= MPM_F_EXX;
-        if (aux && !(cfg->flags & MPM_HIP_STORE_AUX)) {
-            ctx->pfield[f] = nullptr;
-            continue;
-        }
+    for (int f = 0; f < MPM_F_EXX; ++f) {
         ALLOC_OR_FAIL(ctx->pfield[f], np * sizeof(double));
     }
     Particles &P = ctx->P;
-    P.x = ctx->pfield[MPM_F_X];
-    P.y = ctx->pfield[MPM_F_Y];
-    P.vx = ctx->pfield[MPM_F_VX];
-    P.vy = ctx->pfield[MPM_F_VY];
-    P.m = ctx->pfield[MPM_F_M];
-    P.v = ctx->pfield[MPM_F_V];
-    P.txx = ctx->pfield[MPM_F_TXX];
-    P.txy = ctx->pfield[MPM_F_TXY];
-    P.tyy = ctx->pfield[MPM_F_TYY];
-    P.tzz = ctx->pfield[MPM_F_TZZ];
-    P.gp = ctx->pfield[MPM_F_GAMMAP];
-    P.gdp = ctx->pfield[MPM_F_GAMMADOTP];
-    P.ux = ctx->pfield[MPM_F_UX];
-    P.uy = ctx->pfield[MPM_F_UY];
-    P.exx = ctx->pfield[MPM_F_EXX];
-    P.exy = ctx->pfield[MPM_F_EXY];
-    P.eyy = ctx->pfield[MPM_F_EYY];
-    P.wxy = ctx->pfield[MPM_F_WXY];
-    P.ax = ctx->pfield[MPM_F_AX];
-    P.ay = ctx->pfield[MPM_F_AY];
-    P.xl = ctx->pfield[MPM_F_XL];
-    P.yl = ctx->pfield[MPM_F_YL];
+    bind_fields(ctx);
+    if ((cfg->flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) && alloc_aux(ctx) != MPM_HIP_OK) {
+        mpm_hip_destroy(ctx);
+        return MPM_HIP_ERR_ALLOC;
+    }
     ALLOC_OR_FAIL(P.elem, np * sizeof(int));
     ALLOC_OR_FAIL(P.active, np * sizeof(int));
     ALLOC_OR_FAIL(ctx->nodes, nn * sizeof(NodeRec));
@@ -864,7 +1026,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     DevState s0;
     memset(&s0, 0, sizeof s0);
     s0.t = cfg->t0;
-    s0.t0 = cfg->t0;
+    s0.t0 = cfg->loads_t0;
     if (hipMemcpyAsync(ctx->st, &s0, sizeof s0, hipMemcpyHostToDevice, ctx->stream) != hipSuccess
         || hipStreamSynchronize(ctx->stream) != hipSuccess) {
         mpm_hip_destroy(ctx);
@@ -916,7 +1078,7 @@ int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
     DevState s0;
     memset(&s0, 0, sizeof s0);
     s0.t = ctx->cfg.t0;
-    s0.t0 = ctx->cfg.t0;
+    s0.t0 = ctx->cfg.loads_t0;
     HIP_TRY(hipMemcpyAsync(ctx->st, &s0, sizeof s0, hipMemcpyHostToDevice, ctx->stream));
     rc = rebin(ctx, 1);
     if (rc) return rc;
@@ -996,6 +1158,36 @@ int mpm_hip_set_bc_mask(mpm_hip_ctx *ctx, const uint8_t *mask)
     if (rc) return rc;
     const size_t nn = (size_t)ctx->g.N * ctx->g.N;
     HIP_TRY(hipMemcpyAsync(ctx->mask, mask, nn, hipMemcpyHostToDevice, ctx->stream));
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_set_body_force(mpm_hip_ctx *ctx, double bx, double by)
+{
+    if (!ctx) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const double b[2] = {bx, by};
+    HIP_TRY(hipMemcpyAsync((char *)ctx->st + offsetof(DevState, bx), b, sizeof b,
+                           hipMemcpyHostToDevice, ctx->stream));
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_set_store_aux(mpm_hip_ctx *ctx, int on)
+{
+    if (!ctx) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    if (on) {
+        rc = alloc_aux(ctx);
+        if (rc) return rc;
+        ctx->cfg.flags |= MPM_HIP_STORE_AUX;
+    } else {
+        ctx->cfg.flags &= ~MPM_HIP_STORE_AUX;
+    }
     return mpm_hip_sync(ctx);
 }
 

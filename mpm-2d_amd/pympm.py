@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpm_hip.so")
 # mpm_hip_bc_kind
 BC_NONE, BC_BOX, BC_SQUARE_SILO, BC_MASK = 0, 1, 2, 3
 STORE_AUX = 0x1
+HOST_MATERIAL = 0x2
 
 # mpm_hip_field
 FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
@@ -44,7 +45,8 @@ class Material(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("h", ctypes.c_double), ("dt", ctypes.c_double),
-                ("t0", ctypes.c_double), ("g_mag", ctypes.c_double),
+                ("t0", ctypes.c_double), ("loads_t0", ctypes.c_double),
+                ("g_mag", ctypes.c_double),
                 ("ramp_time", ctypes.c_double), ("material", Material),
                 ("bc_kind", ctypes.c_int), ("bc_fp64", ctypes.c_double * 4),
                 ("bc_int", ctypes.c_int * 4), ("bin_capacity", ctypes.c_int),
@@ -73,6 +75,8 @@ def load():
     lib.mpm_hip_get_node_field.argtypes = [P, I, D]
     lib.mpm_hip_set_bc_mask.argtypes = [P, ctypes.POINTER(ctypes.c_uint8)]
     lib.mpm_hip_step.argtypes = [P, I]
+    lib.mpm_hip_set_body_force.argtypes = [P, ctypes.c_double, ctypes.c_double]
+    lib.mpm_hip_set_store_aux.argtypes = [P, I]
     lib.mpm_hip_sync.argtypes = [P]
     lib.mpm_hip_get_time.argtypes = [P, D, D]
     lib.mpm_hip_get_crossings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
@@ -101,7 +105,7 @@ def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
     lib = load()
     cfg = Config()
     lib.mpm_hip_default_config(ctypes.byref(cfg))
-    cfg.N, cfg.h, cfg.dt, cfg.t0 = N, h, dt, t0
+    cfg.N, cfg.h, cfg.dt, cfg.t0, cfg.loads_t0 = N, h, dt, t0, t0
     cfg.g_mag, cfg.ramp_time = g_mag, ramp_time
     for k, val in (material or {}).items():
         setattr(cfg.material, k, val)
@@ -130,6 +134,12 @@ class Sim:
         stress = [None if s is None else np.ascontiguousarray(s, dtype=np.float64) for s in stress]
         ptrs = [_dptr(a) for a in arrs] + [None if s is None else _dptr(s) for s in stress]
         _check(self.lib.mpm_hip_set_particles(self.ctx, *ptrs), "mpm_hip_set_particles")
+
+    def set_body_force(self, bx, by):
+        _check(self.lib.mpm_hip_set_body_force(self.ctx, bx, by), "mpm_hip_set_body_force")
+
+    def set_store_aux(self, on):
+        _check(self.lib.mpm_hip_set_store_aux(self.ctx, 1 if on else 0), "mpm_hip_set_store_aux")
 
     def step(self, n=1):
         _check(self.lib.mpm_hip_step(self.ctx, n), "mpm_hip_step")

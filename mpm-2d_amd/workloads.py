@@ -104,3 +104,17 @@ def small_box(cells=16, bx=5, by=9, seed=7, material=None, **kw):
              **BOX_FRICTION, particles=block(bx, by, cells, seed=seed))
     d.update(kw)
     return d
+
+
+def write_reference_inputs(w, grid_path, particle_path):
+    """Writes the workload in the reference's grid and particle file formats
+    (src/reader.c:17-119), with 17 significant digits so every double
+    survives the round trip."""
+    with open(grid_path, "w") as f:
+        f.write(f"{w['N']}\n1\n")
+    p = w["particles"]
+    cols = [p[k] for k in ("m", "v", "x", "y", "vx", "vy", "sxx", "sxy", "syy")]
+    with open(particle_path, "w") as f:
+        f.write(f"{len(p['x'])}\n")
+        for row in zip(*cols):
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")

@@ -151,8 +151,8 @@ def test_particles_leaving_the_grid():
     parts = {k: v.copy() for k, v in w["particles"].items()}
     parts["x"] += 0.02
     parts["y"] += 0.02
-    parts["vx"][:] = -2.0
-    parts["vy"][::3] = -1.0
+    parts["vx"][:] = -300.0        # 0.12 in 40 steps of 1e-5: many leave
+    parts["vy"][::3] = -100.0
     sim, o = run_both(w, w["steps"], particles=parts)
     rp = o.particles()
     assert 0 < rp["active"].sum() < len(rp["active"])
