@@ -52,8 +52,10 @@ constexpr int kRegX = kTileX + 1;  // cells whose particles reach the tile: 17 x
 constexpr int kRegY = kTileY + 1;
 constexpr int kRegCells = kRegX * kRegY;
 constexpr int kColourCellsMax = ((kRegX + 1) / 2) * ((kRegY + 1) / 2);   // 9 x 5
-constexpr int kStageMax = 720;     // staged particles per batch (64 B each)
-constexpr int kP2GThreads = kTileX * kTileY;
+constexpr int kStageMax = 704;     // staged records per batch (64 B each, incl. one
+                                   // padding record per cell)
+constexpr int kTileNodes = kTileX * kTileY;
+constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
 constexpr int kG2PThreads = 256;
 
 enum : int { kStatusBinOverflow = 1 };
@@ -348,15 +350,19 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // bin entries of one colour's cells held per thread between colours
     constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
     constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;
+    // bins are stored CAP + 1 apart so the per-cell sort threads hit
+    // different LDS banks
+    constexpr int kLstStride = CAP + 1;
 
     __shared__ int s_cnt[kRegCells];
-    __shared__ int s_lst[kColourCellsMax * CAP];
+    __shared__ int s_lst[kColourCellsMax * kLstStride];
     __shared__ int s_off[kColourCellsMax + 1];
-    // staged particles of one colour, as pairs for 16-byte LDS reads
-    __shared__ double2 s_xy[kStageMax];    // local coordinates
-    __shared__ double2 s_mpx[kStageMax];   // m, vx * m
-    __shared__ double2 s_pys[kStageMax];   // vy * m, -sxx * v
-    __shared__ double2 s_ss[kStageMax];    // -sxy * v, -syy * v
+    __shared__ unsigned char s_owner[kColourCellsMax * kLstStride];   // staged slot -> cell
+    // staged particles of one colour, paired for 16-byte LDS reads
+    __shared__ double2 s_xy[kStageMax];     // local coordinates xl, yl
+    __shared__ double2 s_msy[kStageMax];    // m, -sxy * v
+    __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
+    __shared__ double2 s_pys[kStageMax];    // vy * m, -syy * v
 
     // Consecutive tiles go to the same XCD (workgroups are dealt to the 8
     // XCDs round-robin), so neighbouring tiles share that XCD's L2.
@@ -369,10 +375,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const int ty0 = (tile / tiles_x) * kTileY;
     const int cx0 = tx0 - 1, cy0 = ty0 - 1;   // staged cell region origin
     const int tid = threadIdx.x;
+    // Two threads per node: role 0 folds the m, mx, fx chains, role 1 the
+    // my, fy chains (each chain is one node field, summed in the reference's
+    // order by exactly one thread).
+    const int role = tid / kTileNodes;
+    const int nt = tid % kTileNodes;
 
     // This tile owns cells (tx0.., ty0..): clear their next-step bin counts.
-    {
-        const int cc = tx0 + (tid % kTileX), rr = ty0 + (tid / kTileX);
+    if (role == 0) {
+        const int cc = tx0 + (nt % kTileX), rr = ty0 + (nt / kTileX);
         if (cc < g.Ne && rr < g.Ne) {
             cnt_next[rr * g.Ne + cc] = 0;
         }
@@ -414,13 +425,15 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     };
     fetch_lists(0);
 
-    const int i = tx0 + (tid % kTileX);
-    const int j = ty0 + (tid / kTileX);
+    const int i = tx0 + (nt % kTileX);
+    const int j = ty0 + (nt / kTileX);
     const bool live = i < g.N && j < g.N;
-    const double bx = st->bx, by = st->by;
-    double am = 0.0, amx = 0.0, amy = 0.0, afx = 0.0, afy = 0.0;
+    const double bb = role == 0 ? st->bx : st->by;   // body force of this role's chain
+    double am = 0.0, ap = 0.0, af = 0.0;             // role 0: m, mx, fx; role 1: -, my, fy
 
     // Colours in the reference's order; colour = 2 * (row % 2) + (col % 2).
+    // Unrolled, so each colour's cell geometry is a compile-time constant.
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int pc = c & 1, pr = c >> 1;
         const int ncx = ncx_of(c), ncy = ncy_of(c);
@@ -430,32 +443,51 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         for (int u = 0; u < kListSlots; ++u) {
             const int idx = tid + u * kP2GThreads;
             if (idx < kColourCellsMax * CAP && lst_reg[u] >= 0) {
-                s_lst[idx] = lst_reg[u];
+                s_lst[(idx / CAP) * kLstStride + idx % CAP] = lst_reg[u];
             }
         }
         __syncthreads();
-        if (tid < nq) {
-            // sort the bin by particle index (insertion sort; bins arrive
+        if (tid < 64) {
+            // staging offsets: an exclusive scan of (count + 1) over the
+            // colour's cells in the first wave (the extra slot per cell
+            // staggers the cells' staged records across LDS banks), and the
+            // owner cell of every staged slot
+            int n = 0;
+            if (tid < nq) {
+                n = s_cnt[(2 * (tid / ncx) + cb) * kRegX + 2 * (tid % ncx) + ca] + 1;
+            }
+            int incl = n;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int up = __shfl_up(incl, d);
+                if (tid >= d) incl += up;
+            }
+            if (tid < nq) {
+                s_off[tid + 1] = incl;
+                for (int t = incl - n; t < incl; ++t) {
+                    s_owner[t] = (unsigned char)tid;
+                }
+            }
+            if (tid == 0) {
+                s_off[0] = 0;
+            }
+        }
+        if (tid >= kP2GThreads - kColourCellsMax && tid - (kP2GThreads - kColourCellsMax) < nq) {
+            // sort one bin by particle index (insertion sort; bins arrive
             // nearly sorted because lanes of a wave insert in lane order)
-            const int a = 2 * (tid % ncx) + ca, b = 2 * (tid / ncx) + cb;
+            const int q = tid - (kP2GThreads - kColourCellsMax);
+            const int a = 2 * (q % ncx) + ca, b = 2 * (q / ncx) + cb;
             const int n = s_cnt[b * kRegX + a];
-            int *lst = &s_lst[tid * CAP];
+            int *lst = &s_lst[q * kLstStride];
             for (int k = 1; k < n; ++k) {
                 const int key = lst[k];
-                int q = k - 1;
-                while (q >= 0 && lst[q] > key) {
-                    lst[q + 1] = lst[q];
-                    --q;
+                int r = k - 1;
+                while (r >= 0 && lst[r] > key) {
+                    lst[r + 1] = lst[r];
+                    --r;
                 }
-                lst[q + 1] = key;
+                lst[r + 1] = key;
             }
-        } else if (tid == kP2GThreads - 1) {
-            int off = 0;
-            for (int q = 0; q < nq; ++q) {
-                s_off[q] = off;
-                off += s_cnt[(2 * (q / ncx) + cb) * kRegX + 2 * (q % ncx) + ca];
-            }
-            s_off[nq] = off;
         }
         __syncthreads();
         if (c < 3) {
@@ -474,7 +506,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
         }
 
-        // Stage the colour's particles in batches of whole cells.
+        // Stage the colour's particles in batches of whole cells.  Cell q's
+        // records sit at [s_off[q], s_off[q+1] - 1); slot s_off[q+1] - 1 is
+        // padding.
         int qb = 0;
         while (qb < nq) {
             const int base = s_off[qb];
@@ -491,13 +525,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 sc[u] = -1;
                 if (s < nb) {
                     const int gi = base + s;
-                    int lo = qb, hi = qe - 1;      // last cell starting at or before gi
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_off[mid] <= gi) lo = mid; else hi = mid - 1;
+                    const int lo = s_owner[gi];
+                    const int k = gi - s_off[lo];
+                    if (k < s_off[lo + 1] - s_off[lo] - 1) {   // not the padding slot
+                        sp[u] = s_lst[lo * kLstStride + k];
+                        sc[u] = lo;
                     }
-                    sp[u] = s_lst[lo * CAP + (gi - s_off[lo])];
-                    sc[u] = lo;
                 }
             }
             double px[kStageSlots], py[kStageSlots], pm[kStageSlots], pv[kStageSlots];
@@ -525,40 +558,56 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                     const double mp = pm[u], vp = pv[u];
                     s_xy[s] = make_double2(snap_local((px[u] - (double)cc * g.h) / g.h),
                                            snap_local((py[u] - (double)rr * g.h) / g.h));
-                    s_mpx[s] = make_double2(mp, pvx[u] * mp);     // pdata[0..1]
-                    s_pys[s] = make_double2(pvy[u] * mp, -psx[u] * vp);
-                    s_ss[s] = make_double2(-psy[u] * vp, -psz[u] * vp);
+                    // pdata = {m, x_t m, y_t m, ..}, stressdata = {-sxx v, -sxy v, -syy v}
+                    s_msy[s] = make_double2(mp, -psy[u] * vp);
+                    s_pxs[s] = make_double2(pvx[u] * mp, -psx[u] * vp);
+                    s_pys[s] = make_double2(pvy[u] * mp, -psz[u] * vp);
                 }
             }
             __syncthreads();
             if (myq >= qb && myq < qe) {
-                const int k1 = s_off[myq + 1] - base;
+                const int k1 = s_off[myq + 1] - 1 - base;
+                const double2 *s_mom = role == 0 ? s_pxs : s_pys;
                 for (int s = s_off[myq] - base; s < k1; ++s) {
-                    const double2 xy = s_xy[s], mpx = s_mpx[s], pys = s_pys[s], ss = s_ss[s];
+                    const double2 xy = s_xy[s], msy = s_msy[s], ps = s_mom[s];
                     const double X = dx ? xy.x : 1.0 - xy.x;
                     const double Y = dy ? xy.y : 1.0 - xy.y;
                     const double sf = X * Y;
                     const double b1 = (dx ? Y : -Y) * g.inv_h;
                     const double b2 = (dy ? X : -X) * g.inv_h;
-                    const double mp = mpx.x;
-                    am += sf * mp;
-                    amx += sf * mpx.y;
-                    amy += sf * pys.x;
-                    afx += sf * (bx * mp);
-                    afx += b1 * pys.y;
-                    afx += b2 * ss.x;
-                    afy += sf * (by * mp);
-                    afy += b1 * ss.x;
-                    afy += b2 * ss.y;
+                    const double mp = msy.x;
+                    if (role == 0) {
+                        am += sf * mp;
+                        ap += sf * ps.x;
+                        af += sf * (bb * mp);
+                        af += b1 * ps.y;       // -sxx v
+                        af += b2 * msy.y;      // -sxy v
+                    } else {
+                        ap += sf * ps.x;
+                        af += sf * (bb * mp);
+                        af += b1 * msy.y;      // -sxy v
+                        af += b2 * ps.y;       // -syy v
+                    }
                 }
             }
             __syncthreads();
             qb = qe;
         }
     }
-    if (!live) {
+
+    // Hand the y chains to the role-0 thread of the node (staging space is
+    // free after the last barrier).
+    double2 *s_y = s_xy;
+    if (role == 1) {
+        s_y[nt] = make_double2(ap, af);
+    }
+    __syncthreads();
+    if (role != 0 || !live) {
         return;
     }
+    double amx = ap, afx = af;
+    const double2 yv = s_y[nt];
+    double amy = yv.x, afy = yv.y;
 
     // bc_momentum then bc_force, then move_grid_split.
     const unsigned f = bc_fixed(g, i, j, st->t + g.dt, mask);
