@@ -89,6 +89,26 @@ typedef struct mpm_hip_material {
                                   writes TXX/TXY/TYY (the plugin's sxx, sxy,
                                   syy) back before the next step */
 
+/* g_nonlocal constants (materialsrc/g_nonlocal.c:31-47, 141, 313). */
+typedef struct mpm_hip_nonlocal {
+    double E;               /* reference EMOD 1e8 */
+    double nu;              /* reference NUMOD 0.3 */
+    double mu_s;            /* reference 0.3819 */
+    double A;               /* nonlocal amplitude, reference 0.48 */
+    double B;               /* reference 0.9377 * 20 */
+    double d;               /* grain diameter, reference 0.01 * 5 */
+    double rho;             /* grain density, reference 2450 */
+    double p_cap;           /* pressure floor of the bulk fluidity, 1e-2 */
+    double c;               /* cohesion-like pressure cut-off, 1e-2 */
+} mpm_hip_nonlocal;
+
+enum mpm_hip_material_kind {
+    MPM_MAT_G_LOCAL_MU2 = 0,   /* materialsrc/g_local_mu2.c */
+    MPM_MAT_G_NONLOCAL = 1     /* materialsrc/g_nonlocal.c: per-step FEM
+                                  fluidity solve, here Jacobi-preconditioned
+                                  conjugate gradients on the device */
+};
+
 typedef struct mpm_hip_config {
     int N;                  /* nodes per side; grid covers [0,1]^2 */
     double h;               /* node spacing, len / (N - 1) */
@@ -104,6 +124,10 @@ typedef struct mpm_hip_config {
     int bin_capacity;       /* max particles per cell; 0 = 32 */
     int flags;
     int device;             /* HIP device ordinal */
+    int material_kind;      /* mpm_hip_material_kind */
+    mpm_hip_nonlocal nonlocal;
+    double cg_rel_tol;      /* g_nonlocal solve: stop at |r| <= tol |f| */
+    int cg_max_iter;        /* g_nonlocal solve: iteration cap */
 } mpm_hip_config;
 
 typedef struct mpm_hip_ctx mpm_hip_ctx;
@@ -118,6 +142,9 @@ typedef enum mpm_hip_field {
     /* auxiliary outputs, valid only with MPM_HIP_STORE_AUX */
     MPM_F_EXX, MPM_F_EXY, MPM_F_EYY, MPM_F_WXY, MPM_F_AX, MPM_F_AY,
     MPM_F_XL, MPM_F_YL,
+    /* g_nonlocal state (valid with MPM_MAT_G_NONLOCAL): state[3] nonlocal
+     * fluidity, state[5] local (bulk) fluidity, state[10] xi^-2 */
+    MPM_F_GF, MPM_F_GF_BULK, MPM_F_XISQ_INV,
     MPM_F_COUNT
 } mpm_hip_field;
 
@@ -176,6 +203,9 @@ int mpm_hip_get_time(mpm_hip_ctx *ctx, double *t, double *by);
 /* Number of element changes seen since creation (the reference logs one
  * line per change, libmpm/process_usl.c:651-663). */
 int mpm_hip_get_crossings(mpm_hip_ctx *ctx, long long *count);
+
+/* g_nonlocal: iterations and final relative residual of the last solve. */
+int mpm_hip_get_solver_stats(mpm_hip_ctx *ctx, int *iterations, double *rel_residual);
 
 /* Runs n steps and times them with HIP events recorded on the context's
  * stream.  ms[0] = whole span, ms[1] = sum of P2G+grid kernel times,

@@ -31,6 +31,7 @@
 // interpolate.c:23-65) are recomputed where needed from the same inputs with
 // the same operations, so both kernels see identical values.  Everything is
 // FP64 and compiled with -ffp-contract=off; exp is the IEEE-only mpm_exp.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -59,7 +60,7 @@ constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
 constexpr int kG2PThreads = 256;
 
 enum : int { kStatusBinOverflow = 1 };
-enum : int { kModeStoreAux = 1, kModeHostMaterial = 2 };
+enum : int { kModeStoreAux = 1, kModeHostMaterial = 2, kModeNonlocal = 4 };
 
 struct DevState {
     double t;           // job->t
@@ -89,14 +90,18 @@ struct GridParams {
 struct Material {
     double G, two_G, lambda;
     double mu_s, mu_2, I_0, grains_d, grains_rho, rho_crit;
+    // g_nonlocal (materialsrc/g_nonlocal.c:31-47)
+    double nl_E, nl_nu, nl_mu_s, nl_A, nl_B, nl_d, nl_rho, nl_p_cap, nl_c;
 };
 
 struct Particles {
     double *x, *y, *vx, *vy, *m, *v;
     double *txx, *txy, *tyy, *tzz, *gp, *gdp, *ux, *uy;
     double *exx, *exy, *eyy, *wxy, *ax, *ay, *xl, *yl;   // auxiliary
+    double *gf, *gf_bulk, *xisq_inv;                     // g_nonlocal state
     int *elem;          // element of the next step, -1 outside the grid
     int *active;
+    int *elem_prev;     // g_nonlocal: element of the step just taken
 };
 
 struct alignas(64) NodeRec {
@@ -306,6 +311,28 @@ __device__ __forceinline__ void bin_insert(bool valid, int key, int p, int *cnt,
         }
         pending &= ~group;
     }
+}
+
+// calculate_bulk_granular_fluidity (g_nonlocal.c:306-352) for one particle:
+// local fluidity gf_bulk and xi^-2 from the stress at the start of the
+// material update.
+__device__ __forceinline__ void nonlocal_bulk(double sxx, double sxy, double syy,
+                                              const Material &mt, double &gf_bulk,
+                                              double &xisq_inv)
+{
+    double p_t = -0.5 * (sxx + syy);
+    const double sxx_tr0 = sxx + p_t;
+    const double sxy_tr0 = sxy;
+    const double syy_tr0 = syy + p_t;
+    const double tau_t = sqrt(0.5 * (sxx_tr0 * sxx_tr0 + 2 * sxy_tr0 * sxy_tr0
+                                     + syy_tr0 * syy_tr0));
+    if (p_t < mt.nl_p_cap) {
+        p_t = mt.nl_p_cap;
+    }
+    const double S0 = p_t * mt.nl_mu_s;
+    const double eta = mt.nl_B * mt.nl_d * sqrt(p_t * mt.nl_rho);
+    xisq_inv = fabs(tau_t / p_t - mt.nl_mu_s) / (mt.nl_d * mt.nl_d * mt.nl_A * mt.nl_A);
+    gf_bulk = tau_t > S0 ? (p_t / eta) * (1 - S0 / tau_t) : 0.0;
 }
 
 __device__ __forceinline__ double gravity(double t, double t0, double g_mag, double ramp)
@@ -713,7 +740,14 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     const double mp = P.m[p];
     const double v = P.v[p] * mpm_exp(dt * (exx + eyy));
 
-    if (!(mode & kModeHostMaterial)) {
+    if (mode & kModeNonlocal) {
+        // the stress update waits for the fluidity solve; keep what it needs
+        double gfb, xsi;
+        nonlocal_bulk(P.txx[p], P.txy[p], P.tyy[p], mt, gfb, xsi);
+        P.gf_bulk[p] = gfb;
+        P.xisq_inv[p] = xsi;
+        P.elem_prev[p] = e;
+    } else if (!(mode & kModeHostMaterial)) {
         double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
         double gp = P.gp[p], gdp;
         g_local_mu2_update(exx, exy, eyy, wxy, mp, v, dt, mt, txx, txy, tyy, tzz, gp, gdp);
@@ -752,6 +786,466 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     bin_insert(en >= 0, en, p, cnt_next, bkt_next, g.cap, &st->status);
 }
 
+// ---------------------------------------------------------------------------
+// g_nonlocal: fluidity system, solve and stress update
+// ---------------------------------------------------------------------------
+
+struct NlSystem {
+    double *K;          // 9-point stencil, [9][N*N], slot (oy+1)*3 + (ox+1)
+    double *f, *x, *r, *p, *dinv;
+    double *u, *w, *s;  // [2][N*N] each: previous and next iterate
+    double *partials;   // [2][4][kNlMaxBlocks]
+    unsigned int *barrier;   // grid barrier counter of the solve
+    int *stats;         // iterations
+    double *res;        // final relative residual
+    int N;
+};
+
+constexpr int kNlThreads = 256;
+constexpr int kCgThreads = 1024;
+constexpr int kNlMaxBlocks = 2048;
+
+// Assembly of K g = f (solve_diffusion_part, g_nonlocal.c:356-592) at the
+// nodes: K_ij = sum_p v_p (xi_p^-2 N_i N_j + grad N_i . grad N_j),
+// f_i = sum_p v_p g_loc,p xi_p^-2 N_i, over the particles of the step just
+// taken (their step-start cells and local coordinates).  Unknowns are the
+// nodes with m > TOL; other nodes get the row of the identity.  One thread
+// per node, 16x16 nodes per workgroup; the cells' particle lists are staged
+// and sorted in LDS and folded in colour order, ascending particle index,
+// so the sums are deterministic.
+template <int CAP>
+__global__ __launch_bounds__(kNlThreads) void k_nl_assemble(
+    Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    const NodeRec *__restrict__ nodes, NlSystem S, GridParams g, int tiles_x, int ntiles)
+{
+    constexpr int T = 16, R = T + 1;
+    __shared__ int s_cnt[R * R];
+    __shared__ int s_lst[R * R * CAP];
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) {
+        return;
+    }
+    if (tile == 0 && threadIdx.x == 0) {
+        *S.barrier = 0u;   // the solve that follows starts a fresh barrier count
+    }
+    const int tx0 = (tile % tiles_x) * T, ty0 = (tile / tiles_x) * T;
+    const int cx0 = tx0 - 1, cy0 = ty0 - 1;
+    const int tid = threadIdx.x;
+    for (int cl = tid; cl < R * R; cl += kNlThreads) {
+        const int cc = cx0 + cl % R, rr = cy0 + cl / R;
+        int n = 0;
+        if (cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
+            n = cnt_cur[rr * g.Ne + cc];
+            n = n < CAP ? n : CAP;
+        }
+        s_cnt[cl] = n;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < R * R * CAP; idx += kNlThreads) {
+        const int cl = idx / CAP, k = idx % CAP;
+        if (k < s_cnt[cl]) {
+            const int cc = cx0 + cl % R, rr = cy0 + cl / R;
+            s_lst[idx] = bkt_cur[(size_t)(rr * g.Ne + cc) * CAP + k];
+        }
+    }
+    __syncthreads();
+    for (int cl = tid; cl < R * R; cl += kNlThreads) {
+        int *a = &s_lst[cl * CAP];
+        const int n = s_cnt[cl];
+        for (int k = 1; k < n; ++k) {
+            const int key = a[k];
+            int q = k - 1;
+            while (q >= 0 && a[q] > key) {
+                a[q + 1] = a[q];
+                --q;
+            }
+            a[q + 1] = key;
+        }
+    }
+    __syncthreads();
+
+    const int i = tx0 + tid % T, j = ty0 + tid / T;
+    if (i >= g.N || j >= g.N) {
+        return;
+    }
+    const size_t nn = (size_t)g.N * g.N;
+    const size_t me = (size_t)j * g.N + i;
+    auto is_dof = [&](int ii, int jj) { return nodes[(size_t)jj * g.N + ii].m > kTol; };
+    double Kc[9];
+#pragma unroll
+    for (int o = 0; o < 9; ++o) {
+        Kc[o] = 0.0;
+    }
+    double fi = 0.0;
+    const bool dof = is_dof(i, j);
+    if (dof) {
+        for (int c = 0; c < 4; ++c) {
+            const int cc = i - ((i ^ c) & 1);
+            const int rr = j - ((j ^ (c >> 1)) & 1);
+            if (cc < 0 || rr < 0 || cc >= g.Ne || rr >= g.Ne) {
+                continue;
+            }
+            const int dx = i - cc, dy = j - rr;
+            const int cl = (cc - cx0) + (rr - cy0) * R;
+            const int n = s_cnt[cl];
+            // corners b = BL, BR, TR, TL of this cell; skip non-unknowns
+            const bool db[4] = {is_dof(cc, rr), is_dof(cc + 1, rr), is_dof(cc + 1, rr + 1),
+                                is_dof(cc, rr + 1)};
+            double kb[4] = {0.0, 0.0, 0.0, 0.0};
+            double fc = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const int p = s_lst[cl * CAP + k];
+                const double xl = P.xl[p], yl = P.yl[p];
+                const double vp = P.v[p], xsi = P.xisq_inv[p], gfb = P.gf_bulk[p];
+                const double ox = 1.0 - xl, oy = 1.0 - yl;
+                const double sb[4] = {ox * oy, xl * oy, xl * yl, ox * yl};
+                const double gxb[4] = {-oy * g.inv_h, oy * g.inv_h, yl * g.inv_h, -yl * g.inv_h};
+                const double gyb[4] = {-ox * g.inv_h, -xl * g.inv_h, xl * g.inv_h, ox * g.inv_h};
+                const int a = dy ? (dx ? 2 : 3) : (dx ? 1 : 0);
+                const double sa = sb[a], gxa = gxb[a], gya = gyb[a];
+                fc += ((vp * gfb) * xsi) * sa;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (db[b]) {
+                        kb[b] += vp * (xsi * sa * sb[b] + (gxa * gxb[b] + gya * gyb[b]));
+                    }
+                }
+            }
+            fi += fc;
+            // corner b sits at offset (bx - dx, by - dy) from this node
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int bx = (b == 1 || b == 2) ? 1 : 0, by = (b >= 2) ? 1 : 0;
+                const int o = (by - dy + 1) * 3 + (bx - dx + 1);
+                Kc[o] += kb[b];
+            }
+        }
+    } else {
+        Kc[4] = 1.0;
+    }
+#pragma unroll
+    for (int o = 0; o < 9; ++o) {
+        S.K[(size_t)o * nn + me] = Kc[o];
+    }
+    S.f[me] = fi;
+    S.dinv[me] = 1.0 / Kc[4];
+    if (!dof) {
+        S.x[me] = 0.0;
+    }
+}
+
+// Sum over a workgroup in a fixed order (per wave: shuffle tree; across
+// waves: LDS, wave 0 adds in order).
+__device__ __forceinline__ double block_sum(double v, double *s_red)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        v += __shfl_down(v, d);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        s_red[w] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        t += s_red[k];
+    }
+    return t;   // valid in every thread
+}
+
+// Sum of the per-workgroup partials, same order in every workgroup.
+__device__ __forceinline__ double grid_total(const double *part, int nblocks, double *s_red)
+{
+    double v = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+        v += part[k];
+    }
+    return block_sum(v, s_red);
+}
+
+// Three workgroup sums at once (s_red holds 3 * waves entries).
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *s_red)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_down(a, d);
+        b += __shfl_down(b, d);
+        c += __shfl_down(c, d);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        s_red[w] = a;
+        s_red[nw + w] = b;
+        s_red[2 * nw + w] = c;
+    }
+    __syncthreads();
+    a = b = c = 0.0;
+    for (int k = 0; k < nw; ++k) {
+        a += s_red[k];
+        b += s_red[nw + k];
+        c += s_red[2 * nw + k];
+    }
+}
+
+// grid_total of three partial arrays spaced kNlMaxBlocks apart.
+__device__ __forceinline__ void grid_total3(const double *part, int nblocks, double &a,
+                                            double &b, double &c, double *s_red)
+{
+    a = b = c = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+        a += part[k];
+        b += part[kNlMaxBlocks + k];
+        c += part[2 * kNlMaxBlocks + k];
+    }
+    block_sum3(a, b, c, s_red);
+}
+
+// Grid-wide barrier for a cooperative launch (all workgroups co-resident):
+// one agent-scope counter that only grows; the workgroup's barrier episode
+// k waits for the count to reach k * gridDim.x.  One release fence before the
+// arrival and one acquire fence after the wait; the polling itself is
+// relaxed, so it does not invalidate the L2 on every probe.  The counter is
+// cleared by the assembly kernel that precedes every solve.
+__device__ __forceinline__ void grid_barrier(unsigned int *count, unsigned int &target)
+{
+    __syncthreads();
+    target += gridDim.x;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Sum of the 9-point stencil row i of K against v.
+__device__ __forceinline__ double stencil_dot(const double *__restrict__ K, const double *v,
+                                              int i, int N, int nn)
+{
+    const int ix = i % N, iy = i / N;
+    double acc = 0.0;
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int jx = ix + ox, jy = iy + oy;
+            if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
+                acc += K[(size_t)((oy + 1) * 3 + (ox + 1)) * nn + i] * v[jy * N + jx];
+            }
+        }
+    }
+    return acc;
+}
+
+// Jacobi-preconditioned conjugate gradients on the assembled stencil in the
+// single-reduction form of Chronopoulos and Gear (J. Comput. Appl. Math. 25,
+// 1989): with u = M^-1 r, w = K u, s = K p and q = M^-1 s, one iteration is
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,
+//   u -= alpha M^-1 s,  w = K u,
+//   gamma' = (r,u), delta' = (w,u), beta = gamma'/gamma,
+//   alpha = gamma' / (delta' - beta gamma' / alpha),
+// so the three inner products (with (r,r) for the stopping test) share one
+// grid barrier.  The product w = K u needs the new u of the neighbours; each
+// thread recomputes it from their previous u, w, s (double-buffered), with
+// the same expression the owner uses, so the values agree bit for bit.  One
+// persistent cooperative launch; every workgroup reduces the same partials
+// in the same order and reaches the same stopping decision.
+__global__ __launch_bounds__(kCgThreads) void k_nl_cg(NlSystem S, double rel_tol, int max_iter)
+{
+    __shared__ double s_red[3 * (kCgThreads / 64)];
+    const int N = S.N;
+    const int nn = N * N;
+    const int nb = gridDim.x;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = nb * blockDim.x;
+    const double *__restrict__ K = S.K;
+    const double *__restrict__ dinv = S.dinv;
+    unsigned int target = 0;
+
+    // r = f - K x, u = M^-1 r, p = s = 0
+    double ff = 0.0;
+    for (int i = gid; i < nn; i += stride) {
+        const double r = S.f[i] - stencil_dot(K, S.x, i, N, nn);
+        S.r[i] = r;
+        S.u[i] = dinv[i] * r;
+        S.p[i] = 0.0;
+        S.s[i] = 0.0;
+        ff += S.f[i] * S.f[i];
+    }
+    grid_barrier(S.barrier, target);
+    // w = K u and the first inner products
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int i = gid; i < nn; i += stride) {
+        const double w = stencil_dot(K, S.u, i, N, nn);
+        S.w[i] = w;
+        const double r = S.r[i], u = S.u[i];
+        g += r * u;
+        dl += w * u;
+        rr += r * r;
+    }
+    double *part = S.partials;   // [2][4][kNlMaxBlocks], by iteration parity
+    {
+        const double a = block_sum(g, s_red), b = block_sum(dl, s_red);
+        const double c = block_sum(rr, s_red), d = block_sum(ff, s_red);
+        if (threadIdx.x == 0) {
+            part[blockIdx.x] = a;
+            part[kNlMaxBlocks + blockIdx.x] = b;
+            part[2 * kNlMaxBlocks + blockIdx.x] = c;
+            part[3 * kNlMaxBlocks + blockIdx.x] = d;
+        }
+    }
+    grid_barrier(S.barrier, target);
+    double gamma = grid_total(part, nb, s_red);
+    double delta = grid_total(part + kNlMaxBlocks, nb, s_red);
+    rr = grid_total(part + 2 * kNlMaxBlocks, nb, s_red);
+    ff = grid_total(part + 3 * kNlMaxBlocks, nb, s_red);
+    const double stop = rel_tol * rel_tol * ff;
+    double alpha = gamma / delta, beta = 0.0;
+    int it = 0;
+    while (it < max_iter && rr > stop) {
+        const int cur = it & 1;
+        const double *uo = S.u + (size_t)cur * nn, *wo = S.w + (size_t)cur * nn;
+        const double *so = S.s + (size_t)cur * nn;
+        double *un = S.u + (size_t)(cur ^ 1) * nn, *wn = S.w + (size_t)(cur ^ 1) * nn;
+        double *sn = S.s + (size_t)(cur ^ 1) * nn;
+        double gl = 0.0, dll = 0.0, rrl = 0.0;
+        for (int i = gid; i < nn; i += stride) {
+            const int ix = i % N, iy = i / N;
+            double w = 0.0;
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const int jx = ix + ox, jy = iy + oy;
+                    if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
+                        const int j = jy * N + jx;
+                        const double sj = wo[j] + beta * so[j];
+                        const double uj = uo[j] - alpha * (dinv[j] * sj);
+                        w += K[(size_t)((oy + 1) * 3 + (ox + 1)) * nn + i] * uj;
+                    }
+                }
+            }
+            const double si = wo[i] + beta * so[i];
+            const double ui = uo[i] - alpha * (dinv[i] * si);
+            const double pi = uo[i] + beta * S.p[i];
+            S.p[i] = pi;
+            S.x[i] += alpha * pi;
+            const double ri = S.r[i] - alpha * si;
+            S.r[i] = ri;
+            sn[i] = si;
+            un[i] = ui;
+            wn[i] = w;
+            gl += ri * ui;
+            dll += w * ui;
+            rrl += ri * ri;
+        }
+        double *pt = part + (size_t)((it + 1) & 1) * 4 * kNlMaxBlocks;
+        block_sum3(gl, dll, rrl, s_red);
+        if (threadIdx.x == 0) {
+            pt[blockIdx.x] = gl;
+            pt[kNlMaxBlocks + blockIdx.x] = dll;
+            pt[2 * kNlMaxBlocks + blockIdx.x] = rrl;
+        }
+        grid_barrier(S.barrier, target);
+        double gamma_new, delta_new;
+        grid_total3(pt, nb, gamma_new, delta_new, rr, s_red);
+        beta = gamma_new / gamma;
+        alpha = gamma_new / (delta_new - beta * gamma_new / alpha);
+        gamma = gamma_new;
+        ++it;
+    }
+    if (gid == 0) {
+        S.stats[0] = it;
+        S.res[0] = ff > 0.0 ? sqrt(rr / ff) : 0.0;
+    }
+}
+
+// Nonlocal fluidity at the particles and the explicit visco-plastic update
+// (g_nonlocal.c:145-301, solve_diffusion_part 538-564), every particle.
+__global__ __launch_bounds__(kG2PThreads) void k_nl_stress(
+    Particles P, const double *__restrict__ gnode, GridParams g, Material mt,
+    const DevState *__restrict__ st, int np)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) {
+        return;
+    }
+    double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
+    double gf = P.gf[p];
+    if (P.active[p]) {
+        const int e = P.elem_prev[p];
+        const int cc = e % g.Ne, rr = e / g.Ne;
+        const double xl = P.xl[p], yl = P.yl[p];
+        const double ox = 1.0 - xl, oy = 1.0 - yl;
+        const size_t n0 = (size_t)rr * g.N + cc;
+        gf = 0.0;
+        gf += gnode[n0] * (ox * oy);
+        gf += gnode[n0 + 1] * (xl * oy);
+        gf += gnode[n0 + g.N + 1] * (xl * yl);
+        gf += gnode[n0 + g.N] * (ox * yl);
+        P.gf[p] = gf;
+    } else {
+        double gfb, xsi;
+        nonlocal_bulk(sxx, sxy, syy, mt, gfb, xsi);
+        P.gf_bulk[p] = gfb;
+        P.xisq_inv[p] = xsi;
+    }
+    const double dt = g.dt;
+    const double E = mt.nl_E, nu = mt.nl_nu, c = mt.nl_c;
+    const double exx = P.exx[p], exy = P.exy[p], eyy = P.eyy[p], wxy = P.wxy[p];
+    double dsjxx = dt * (E / (1 - nu * nu)) * ((exx) + nu * (eyy));
+    double dsjyy = dt * (E / (1 - nu * nu)) * ((eyy) + nu * (exx));
+    dsjxx -= 2 * dt * wxy * sxy;
+    dsjyy += 2 * dt * wxy * sxy;
+    const double p_tr = -0.5 * ((sxx + dsjxx) + (syy + dsjyy));
+    if (p_tr < c) {
+        P.txx[p] = 0.5 * c / mt.nl_mu_s;
+        P.txy[p] = 0.0;
+        P.tyy[p] = 0.5 * c / mt.nl_mu_s;
+        return;
+    }
+    const double p_t = -0.5 * (sxx + syy);
+    const double sxx_t0 = sxx + p_t, sxy_t0 = sxy, syy_t0 = syy + p_t;
+    const double tau_t = sqrt(0.5 * (sxx_t0 * sxx_t0 + 2 * sxy_t0 * sxy_t0 + syy_t0 * syy_t0));
+    const double r2 = sqrt(0.5);
+    const double Npxx = r2 * sxx_t0 / tau_t;
+    const double Npxy = r2 * sxy_t0 / tau_t;
+    const double Npyy = r2 * syy_t0 / tau_t;
+    const double dxx0 = 0.5 * (exx - eyy);
+    const double dxy0 = exy;
+    const double dyy0 = 0.5 * (eyy - exx);
+    const double gammadot = sqrt(2 * (dxx0 * dxx0 + 2 * dxy0 * dxy0 + dyy0 * dyy0));
+    double nup = 0.0, dpxx = 0.0, dpxy = 0.0, dpyy = 0.0;
+    if (p_t > c && gf > 0) {
+        nup = gf * tau_t / p_t;
+        if (nup < 0) nup = 0;
+        if (nup > gammadot) nup = gammadot;
+        dpxx = r2 * (nup) * Npxx;
+        dpxy = r2 * (nup) * Npxy;
+        dpyy = r2 * (nup) * Npyy;
+    }
+    if (st->t == 0) {
+        nup = 0.0;
+        dpxx = dpxy = dpyy = 0.0;
+    }
+    dsjxx = dt * (E / (1 - nu * nu)) * ((exx - dpxx) + nu * (eyy - dpyy));
+    double dsjxy = dt * (E / (2 * (1 + nu))) * (exy - dpxy);
+    dsjyy = dt * (E / (1 - nu * nu)) * ((eyy - dpyy) + nu * (exx - dpxx));
+    dsjxx -= 2 * dt * wxy * sxy;
+    dsjxy += dt * wxy * (sxx - syy);
+    dsjyy += 2 * dt * wxy * sxy;
+    P.gp[p] += nup * dt;
+    P.txx[p] = sxx + dsjxx;
+    P.txy[p] = sxy + dsjxy;
+    P.tyy[p] = syy + dsjyy;
+}
+
 __global__ void k_eval_ops(int op, const double *a, const double *b, double *out, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -788,6 +1282,10 @@ struct mpm_hip_ctx {
     int cur;                // bins holding the current step's lists
     int tiles_x, ntiles, p2g_blocks;
     bool have_particles;
+    // g_nonlocal
+    bool nonlocal;
+    NlSystem nl;
+    int nl_tiles_x, nl_ntiles, cg_blocks;
 };
 
 #define HIP_TRY(expr)                                                              \
@@ -826,6 +1324,9 @@ static void bind_fields(mpm_hip_ctx *ctx)
     P.ay = f[MPM_F_AY];
     P.xl = f[MPM_F_XL];
     P.yl = f[MPM_F_YL];
+    P.gf = f[MPM_F_GF];
+    P.gf_bulk = f[MPM_F_GF_BULK];
+    P.xisq_inv = f[MPM_F_XISQ_INV];
 }
 
 // Auxiliary per-particle outputs, allocated (zeroed) on first use.
@@ -871,18 +1372,53 @@ static int launch_p2g(mpm_hip_ctx *ctx)
     return MPM_HIP_OK;
 }
 
+static int launch_nonlocal(mpm_hip_ctx *ctx, int bins)
+{
+    const size_t nb = ctx->np ? (ctx->np + kG2PThreads - 1) / kG2PThreads : 1;
+    const dim3 ag(ctx->nl_ntiles), ab(kNlThreads);
+    switch (ctx->g.cap) {
+    case 16:
+        k_nl_assemble<16><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+            ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
+        break;
+    case 32:
+        k_nl_assemble<32><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+            ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
+        break;
+    default:
+        k_nl_assemble<64><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+            ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    double tol = ctx->cfg.cg_rel_tol;
+    int max_iter = ctx->cfg.cg_max_iter;
+    void *args[] = {&ctx->nl, &tol, &max_iter};
+    HIP_TRY(hipLaunchCooperativeKernel((const void *)k_nl_cg, dim3(ctx->cg_blocks),
+                                       dim3(kCgThreads), args, 0, ctx->stream));
+    k_nl_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+        ctx->P, ctx->nl.x, ctx->g, ctx->mt, ctx->st, (int)ctx->np);
+    HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
 static int launch_g2p(mpm_hip_ctx *ctx)
 {
     const int nxt = 1 - ctx->cur;
+    const int bins = ctx->cur;   // the step's own bins (the assembly reads them)
     const size_t nb = ctx->np ? (ctx->np + kG2PThreads - 1) / kG2PThreads : 1;
     int mode = 0;
     if (ctx->cfg.flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) mode |= kModeStoreAux;
     if (ctx->cfg.flags & MPM_HIP_HOST_MATERIAL) mode |= kModeHostMaterial;
+    if (ctx->nonlocal) mode |= kModeNonlocal | kModeStoreAux;
     k_g2p_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
         (int)ctx->np, mode);
     HIP_TRY(hipGetLastError());
     ctx->cur = nxt;
+    if (ctx->nonlocal) {
+        return launch_nonlocal(ctx, bins);
+    }
     return MPM_HIP_OK;
 }
 
@@ -921,6 +1457,18 @@ void mpm_hip_default_config(mpm_hip_config *cfg)
     cfg->material.grains_rho = 2450;
     cfg->material.rho_crit = 1485.0f;
     cfg->bc_kind = MPM_BC_NONE;
+    cfg->material_kind = MPM_MAT_G_LOCAL_MU2;
+    cfg->nonlocal.E = 1e8;
+    cfg->nonlocal.nu = 0.3;
+    cfg->nonlocal.mu_s = 0.3819;
+    cfg->nonlocal.A = 0.48;
+    cfg->nonlocal.B = (0.9377 * 20);
+    cfg->nonlocal.d = (0.01 * 5);
+    cfg->nonlocal.rho = 2450;
+    cfg->nonlocal.p_cap = 1e-2;
+    cfg->nonlocal.c = 1e-2;
+    cfg->cg_rel_tol = 1e-10;
+    cfg->cg_max_iter = 20000;
     cfg->bin_capacity = 0;
     cfg->flags = 0;
     cfg->device = 0;
@@ -1025,6 +1573,25 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     mt.grains_d = cfg->material.grains_d;
     mt.grains_rho = cfg->material.grains_rho;
     mt.rho_crit = cfg->material.rho_crit;
+    mt.nl_E = cfg->nonlocal.E;
+    mt.nl_nu = cfg->nonlocal.nu;
+    mt.nl_mu_s = cfg->nonlocal.mu_s;
+    mt.nl_A = cfg->nonlocal.A;
+    mt.nl_B = cfg->nonlocal.B;
+    mt.nl_d = cfg->nonlocal.d;
+    mt.nl_rho = cfg->nonlocal.rho;
+    mt.nl_p_cap = cfg->nonlocal.p_cap;
+    mt.nl_c = cfg->nonlocal.c;
+    if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL) {
+        delete ctx;
+        return MPM_HIP_ERR_INVALID;
+    }
+    ctx->nonlocal = cfg->material_kind == MPM_MAT_G_NONLOCAL;
+    if (ctx->nonlocal && ((cfg->flags & MPM_HIP_HOST_MATERIAL) || !(cfg->cg_rel_tol >= 0.0)
+                          || cfg->cg_max_iter < 0)) {
+        delete ctx;
+        return MPM_HIP_ERR_INVALID;
+    }
 
     ctx->tiles_x = (cfg->N + kTileX - 1) / kTileX;
     ctx->ntiles = ctx->tiles_x * ((cfg->N + kTileY - 1) / kTileY);
@@ -1050,11 +1617,50 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     }
     Particles &P = ctx->P;
     bind_fields(ctx);
-    if ((cfg->flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) && alloc_aux(ctx) != MPM_HIP_OK) {
+    if ((cfg->flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL) || ctx->nonlocal)
+        && alloc_aux(ctx) != MPM_HIP_OK) {
         mpm_hip_destroy(ctx);
         return MPM_HIP_ERR_ALLOC;
     }
     ALLOC_OR_FAIL(P.elem, np * sizeof(int));
+    if (ctx->nonlocal) {
+        ALLOC_OR_FAIL(P.elem_prev, np * sizeof(int));
+        NlSystem &S = ctx->nl;
+        S.N = cfg->N;
+        ALLOC_OR_FAIL(S.K, 9 * nn * sizeof(double));
+        ALLOC_OR_FAIL(S.f, nn * sizeof(double));
+        ALLOC_OR_FAIL(S.x, nn * sizeof(double));
+        ALLOC_OR_FAIL(S.r, nn * sizeof(double));
+        ALLOC_OR_FAIL(S.p, nn * sizeof(double));
+        ALLOC_OR_FAIL(S.u, 2 * nn * sizeof(double));
+        ALLOC_OR_FAIL(S.w, 2 * nn * sizeof(double));
+        ALLOC_OR_FAIL(S.s, 2 * nn * sizeof(double));
+        ALLOC_OR_FAIL(S.barrier, sizeof(unsigned int));
+        ALLOC_OR_FAIL(S.dinv, nn * sizeof(double));
+        ALLOC_OR_FAIL(S.partials, 8 * kNlMaxBlocks * sizeof(double));
+        ALLOC_OR_FAIL(S.stats, sizeof(int));
+        ALLOC_OR_FAIL(S.res, sizeof(double));
+        if (hipMemsetAsync(S.x, 0, nn * sizeof(double), ctx->stream) != hipSuccess
+            || hipMemsetAsync(S.barrier, 0, sizeof(unsigned int), ctx->stream) != hipSuccess
+            || hipMemsetAsync(S.stats, 0, sizeof(int), ctx->stream) != hipSuccess
+            || hipMemsetAsync(S.res, 0, sizeof(double), ctx->stream) != hipSuccess) {
+            mpm_hip_destroy(ctx);
+            return MPM_HIP_ERR_DEVICE;
+        }
+        ctx->nl_tiles_x = (cfg->N + 15) / 16;
+        ctx->nl_ntiles = ctx->nl_tiles_x * ctx->nl_tiles_x;
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_nl_cg,
+                                                         kCgThreads, 0) != hipSuccess || occ < 1) {
+            mpm_hip_destroy(ctx);
+            return MPM_HIP_ERR_DEVICE;
+        }
+        // co-resident workgroups only (the solve synchronises the grid)
+        long want = ((long)nn + kCgThreads - 1) / kCgThreads;
+        long have = (long)occ * prop.multiProcessorCount;
+        long nbk = want < have ? want : have;
+        ctx->cg_blocks = (int)(nbk < kNlMaxBlocks ? nbk : kNlMaxBlocks);
+    }
     ALLOC_OR_FAIL(P.active, np * sizeof(int));
     ALLOC_OR_FAIL(ctx->nodes, nn * sizeof(NodeRec));
     ALLOC_OR_FAIL(ctx->mask, nn);
@@ -1301,6 +1907,23 @@ int mpm_hip_get_crossings(mpm_hip_ctx *ctx, long long *count)
     return MPM_HIP_OK;
 }
 
+int mpm_hip_get_solver_stats(mpm_hip_ctx *ctx, int *iterations, double *rel_residual)
+{
+    if (!ctx || !ctx->nonlocal) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    int it = 0;
+    double res = 0.0;
+    HIP_TRY(hipMemcpyAsync(&it, ctx->nl.stats, sizeof it, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&res, ctx->nl.res, sizeof res, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = res;
+    return MPM_HIP_OK;
+}
+
 int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms)
 {
     if (!ctx || n <= 0 || !ms || !ctx->have_particles) {
@@ -1362,6 +1985,16 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx)
         if (ctx->pfield[f]) hipFree(ctx->pfield[f]);
     }
     if (ctx->P.elem) hipFree(ctx->P.elem);
+    if (ctx->P.elem_prev) hipFree(ctx->P.elem_prev);
+    {
+        NlSystem &S = ctx->nl;
+        double *arrs[] = {S.K, S.f, S.x, S.r, S.p, S.u, S.w, S.s, S.dinv, S.partials, S.res};
+        for (double *a : arrs) {
+            if (a) hipFree(a);
+        }
+        if (S.stats) hipFree(S.stats);
+        if (S.barrier) hipFree(S.barrier);
+    }
     if (ctx->P.active) hipFree(ctx->P.active);
     if (ctx->nodes) hipFree(ctx->nodes);
     if (ctx->mask) hipFree(ctx->mask);

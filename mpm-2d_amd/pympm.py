@@ -20,7 +20,8 @@ HOST_MATERIAL = 0x2
 # mpm_hip_field
 FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
           "gammap", "gammadotp", "ux", "uy",
-          "exx", "exy", "eyy", "wxy", "ax", "ay", "xl", "yl"]
+          "exx", "exy", "eyy", "wxy", "ax", "ay", "xl", "yl",
+          "gf", "gf_bulk", "xisq_inv"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 AUX_FIELDS = FIELDS[14:]
 NODE_FIELDS = ["m", "mx", "my", "fx", "fy", "vx", "vy"]
@@ -43,6 +44,11 @@ class Material(ctypes.Structure):
                 ("grains_rho", ctypes.c_double), ("rho_crit", ctypes.c_double)]
 
 
+class Nonlocal(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("E", "nu", "mu_s", "A", "B", "d", "rho",
+                                                 "p_cap", "c")]
+
+
 class Config(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("h", ctypes.c_double), ("dt", ctypes.c_double),
                 ("t0", ctypes.c_double), ("loads_t0", ctypes.c_double),
@@ -50,7 +56,9 @@ class Config(ctypes.Structure):
                 ("ramp_time", ctypes.c_double), ("material", Material),
                 ("bc_kind", ctypes.c_int), ("bc_fp64", ctypes.c_double * 4),
                 ("bc_int", ctypes.c_int * 4), ("bin_capacity", ctypes.c_int),
-                ("flags", ctypes.c_int), ("device", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("device", ctypes.c_int),
+                ("material_kind", ctypes.c_int), ("nl", Nonlocal),
+                ("cg_rel_tol", ctypes.c_double), ("cg_max_iter", ctypes.c_int)]
 
 
 _lib = None
@@ -81,6 +89,7 @@ def load():
     lib.mpm_hip_get_time.argtypes = [P, D, D]
     lib.mpm_hip_get_crossings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     lib.mpm_hip_time_steps.argtypes = [P, I, D]
+    lib.mpm_hip_get_solver_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_destroy.argtypes = [P]
     lib.mpm_hip_status_string.argtypes = [I]
     lib.mpm_hip_status_string.restype = ctypes.c_char_p
@@ -101,10 +110,18 @@ def _check(code, what):
 
 
 def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
-                g_mag=9.8, ramp_time=0.1, t0=0.0, bin_capacity=0, flags=0, device=0):
+                g_mag=9.8, ramp_time=0.1, t0=0.0, bin_capacity=0, flags=0, device=0,
+                nonlocal_material=None, cg_rel_tol=1e-10, cg_max_iter=20000):
+    """nonlocal_material: g_nonlocal constants (E, nu, mu_s, A, B, d, rho,
+    p_cap, c); selects the g_nonlocal law when given."""
     lib = load()
     cfg = Config()
     lib.mpm_hip_default_config(ctypes.byref(cfg))
+    if nonlocal_material is not None:
+        cfg.material_kind = 1
+        for k, val in nonlocal_material.items():
+            setattr(cfg.nl, k, val)
+    cfg.cg_rel_tol, cfg.cg_max_iter = cg_rel_tol, cg_max_iter
     cfg.N, cfg.h, cfg.dt, cfg.t0, cfg.loads_t0 = N, h, dt, t0, t0
     cfg.g_mag, cfg.ramp_time = g_mag, ramp_time
     for k, val in (material or {}).items():
@@ -186,6 +203,12 @@ class Sim:
         c = ctypes.c_longlong()
         _check(self.lib.mpm_hip_get_crossings(self.ctx, ctypes.byref(c)), "mpm_hip_get_crossings")
         return c.value
+
+    def solver_stats(self):
+        it, res = ctypes.c_int(), ctypes.c_double()
+        _check(self.lib.mpm_hip_get_solver_stats(self.ctx, ctypes.byref(it), ctypes.byref(res)),
+               "mpm_hip_get_solver_stats")
+        return it.value, res.value
 
     def time_steps(self, n):
         ms = np.zeros(3, dtype=np.float64)

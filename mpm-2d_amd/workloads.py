@@ -118,3 +118,38 @@ def write_reference_inputs(w, grid_path, particle_path):
         f.write(f"{len(p['x'])}\n")
         for row in zip(*cols):
             f.write(" ".join(repr(float(v)) for v in row) + "\n")
+
+
+# g_nonlocal constants (materialsrc/g_nonlocal.c:31-47, 141, 313); the heap
+# workload sets the nonlocal amplitude A and mu_s.
+REFERENCE_NONLOCAL = dict(E=1e8, nu=0.3, mu_s=0.3819, A=0.48, B=0.9377 * 20, d=0.01 * 5,
+                          rho=2450.0, p_cap=1e-2, c=1e-2)
+HEAP_NONLOCAL = dict(REFERENCE_NONLOCAL, mu_s=0.38, A=0.48)
+
+
+def lithostatic(p, surface, rho=1500.0, g=9.81, k0=0.4):
+    """Sets a lithostatic stress: syy = -rho g (surface(x) - y), sxx = k0 syy,
+    sxy = 0.  With k0 = 0.4, tau / p = (1 - k0) / (1 + k0) = 0.43 > mu_s, so
+    every particle starts with a nonzero local fluidity."""
+    depth = np.maximum(surface(p["x"]) - p["y"], 0.0)
+    p["syy"] = -rho * g * depth
+    p["sxx"] = k0 * p["syy"]
+    p["sxy"] = np.zeros_like(depth)
+    return p
+
+
+def heap(cells=256, seed=3, **kw):
+    """Nonlocal fluidity heap: cells x cells grid, an isosceles pile with a
+    base of `cells` cells and a height of cells/2 (262,144 particles at 256),
+    g_nonlocal, box with frictionless walls and no-slip floor, dt = 5e-6.
+    The pile starts from the lithostatic state above (same input to the GPU
+    and the CPU step)."""
+    h = 1.0 / cells
+    height, half, xc = (cells // 2) * h, 0.5, 0.5
+    parts = lithostatic(triangle_pile(cells, cells // 2, cells, seed=seed),
+                        lambda x: height * (1.0 - np.abs(x - xc) / half))
+    d = dict(name=f"heap_{cells}", N=cells + 1, h=h, dt=5e-6, steps=500,
+             material=BASELINE_MATERIAL, nonlocal_material=HEAP_NONLOCAL,
+             g_mag=9.81, ramp_time=0.1, **BOX_FRICTION, particles=parts)
+    d.update(kw)
+    return d

@@ -16,7 +16,7 @@ REF_PIECES = os.path.join(HERE, "_ref", "libmpm_ref_pieces.so")
 FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
           "gammap", "gammadotp", "ux", "uy", "exx", "exy", "eyy", "wxy",
           "ax", "ay", "xl", "yl", "sxx", "sxy", "syy", "txz", "tyx", "tyz",
-          "tzx", "tzy", "by"]
+          "tzx", "tzy", "by", "gf", "gf_bulk", "xisq_inv"]
 NODE_FIELDS = ["m", "mx", "my", "fx", "fy", "vx", "vy", "ax", "ay"]
 
 
@@ -30,7 +30,12 @@ class Params(ctypes.Structure):
                 ("g_mag", ctypes.c_double), ("ramp_time", ctypes.c_double),
                 ("bc_kind", ctypes.c_int), ("bc_fp64", ctypes.c_double * 4),
                 ("bc_int", ctypes.c_int * 4), ("log_crossings", ctypes.c_int),
-                ("log_path", ctypes.c_char_p)]
+                ("log_path", ctypes.c_char_p), ("material_kind", ctypes.c_int),
+                ("nl_E", ctypes.c_double), ("nl_nu", ctypes.c_double),
+                ("nl_mu_s", ctypes.c_double), ("nl_A", ctypes.c_double),
+                ("nl_B", ctypes.c_double), ("nl_d", ctypes.c_double),
+                ("nl_rho", ctypes.c_double), ("nl_p_cap", ctypes.c_double),
+                ("nl_c", ctypes.c_double)]
 
 
 _libs = {}
@@ -75,6 +80,10 @@ def load(variant="parity"):
         getattr(lib, fn).argtypes = [D, D] + [ctypes.c_double] * 5
     lib.oracle_tent.argtypes = [D] * 4 + [ctypes.c_double] * 2
     lib.oracle_grad_tent.argtypes = [D] * 8 + [ctypes.c_double] * 3
+    lib.oracle_nonlocal_system.argtypes = [D, D, ctypes.POINTER(ctypes.c_size_t)]
+    lib.oracle_nonlocal_system.restype = ctypes.c_size_t
+    lib.oracle_band_cholesky_solve.argtypes = [D, ctypes.c_size_t, ctypes.c_size_t, D]
+    lib.oracle_band_cholesky_solve.restype = ctypes.c_int
     lib.oracle_negative_root.argtypes = [ctypes.c_double] * 3
     lib.oracle_negative_root.restype = ctypes.c_double
     _libs[variant] = lib
@@ -85,10 +94,20 @@ def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+NONLOCAL_KEYS = ("E", "nu", "mu_s", "A", "B", "d", "rho", "p_cap", "c")
+
+
 def make_params(lib, N, h, dt, material=None, bc_kind=0, bc_fp64=(), bc_int=(),
-                g_mag=9.8, ramp_time=0.1, num_threads=1, log_crossings=False):
+                g_mag=9.8, ramp_time=0.1, num_threads=1, log_crossings=False,
+                nonlocal_material=None):
+    """material: g_local_mu2 constants; nonlocal_material: g_nonlocal constants
+    (keys of NONLOCAL_KEYS) -- selects the g_nonlocal law when given."""
     q = Params()
     lib.oracle_default_params(ctypes.byref(q))
+    if nonlocal_material is not None:
+        q.material_kind = 1
+        for k, val in nonlocal_material.items():
+            setattr(q, "nl_" + k, val)
     q.N, q.h, q.dt, q.num_threads = N, h, dt, num_threads
     for k, val in (material or {}).items():
         setattr(q, k, val)
@@ -135,6 +154,15 @@ class Oracle:
         self.lib.oracle_get_nodes(self.h, _dptr(out))
         return {k: out[i] for i, k in enumerate(NODE_FIELDS)}
 
+    def nonlocal_system(self):
+        """(K lower band [bw+1, n], f [n], bw) of the last nonlocal solve."""
+        bw = ctypes.c_size_t()
+        n = self.lib.oracle_nonlocal_system(None, None, ctypes.byref(bw))
+        band = np.empty((bw.value + 1, n))
+        f = np.empty(n)
+        self.lib.oracle_nonlocal_system(_dptr(band), _dptr(f), ctypes.byref(bw))
+        return band, f, bw.value
+
     def bc_mask(self):
         nn = self.params.N * self.params.N
         out = np.empty(3 * nn, dtype=np.int32)
@@ -165,5 +193,6 @@ def from_workload(w, variant="parity", num_threads=1, log_crossings=False, parti
                     bc_kind=w.get("bc_kind", 0), bc_fp64=w.get("bc_fp64", ()),
                     bc_int=w.get("bc_int", ()), g_mag=w.get("g_mag", 9.8),
                     ramp_time=w.get("ramp_time", 0.1), num_threads=num_threads,
-                    log_crossings=log_crossings)
+                    log_crossings=log_crossings,
+                    nonlocal_material=w.get("nonlocal_material"))
     return Oracle(q, particles if particles is not None else w["particles"], variant=variant)

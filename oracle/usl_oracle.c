@@ -91,6 +91,12 @@ typedef struct oracle_params {
     int bc_int[4];
     int log_crossings;     /* write the reference's per-crossing log lines */
     const char *log_path;  /* NULL: /dev/null when logging */
+    /* material: 0 g_local_mu2, 1 g_nonlocal (materialsrc/g_nonlocal.c) */
+    int material_kind;
+    /* g_nonlocal constants (g_nonlocal.c:31-47, 141, 313):
+     * E (EMOD 1e8), nu (NUMOD 0.3), mu_s (0.3819), A (0.48), B (0.9377*20),
+     * grains_d (0.01*5), grains_rho (2450), p_cap (1e-2), c (1e-2) */
+    double nl_E, nl_nu, nl_mu_s, nl_A, nl_B, nl_d, nl_rho, nl_p_cap, nl_c;
 } oracle_params_t;
 
 typedef struct oracle_job {
@@ -406,6 +412,282 @@ static void g_local_mu2_stress_threaded(threadtask_t *task)
 }
 
 /* ------------------------------------------------------------------------ */
+/* materialsrc/g_nonlocal.c                                                 */
+/* state[3] gf, state[4] eta, state[5] gf_bulk, state[9] gammap,           */
+/* state[10] xisq_inv (g_nonlocal.c:49-59)                                  */
+/* ------------------------------------------------------------------------ */
+
+/* calculate_bulk_granular_fluidity (g_nonlocal.c:306-352): every particle. */
+static void g_nonlocal_bulk(oracle_job_t *oj)
+{
+    job_t *job = &oj->job;
+    const oracle_params_t *q = &oj->prm;
+    const double p_cap = q->nl_p_cap;
+    for (size_t i = 0; i < job->num_particles; i++) {
+        particle_t *pp = &job->particles[i];
+        double p_t = -0.5 * (pp->sxx + pp->syy);
+        double sxx_tr0 = pp->sxx + p_t;
+        double sxy_tr0 = pp->sxy;
+        double syy_tr0 = pp->syy + p_t;
+        double tau_t = sqrt(0.5 * (sxx_tr0 * sxx_tr0 + 2 * sxy_tr0 * sxy_tr0 + syy_tr0 * syy_tr0));
+        if (p_t < p_cap) {
+            p_t = p_cap;
+        }
+        double S0 = p_t * q->nl_mu_s;
+        pp->state[4] = q->nl_B * q->nl_d * sqrt(p_t * q->nl_rho);
+        pp->state[10] = fabs(tau_t / p_t - q->nl_mu_s) / (q->nl_d * q->nl_d * q->nl_A * q->nl_A);
+        if (tau_t > S0) {
+            pp->state[5] = (p_t / pp->state[4]) * (1 - S0 / tau_t);
+        } else {
+            pp->state[5] = 0;
+        }
+    }
+}
+
+/* In-place Cholesky factorisation and solve of a symmetric positive definite
+ * band matrix: band[k * n + i] holds K(i, i - k) for k = 0..bw.  Stands in
+ * for CXSparse's cs_lusol (absent here) in solve_diffusion_part; pinned
+ * against scipy's sparse solver in the tests.  Returns 0 if not SPD. */
+int oracle_band_cholesky_solve(double *band, size_t n, size_t bw, double *b)
+{
+    for (size_t j = 0; j < n; j++) {
+        double d = band[j];
+        size_t kmax = j < bw ? j : bw;
+        for (size_t k = 1; k <= kmax; k++) {
+            double l = band[k * n + j];
+            d -= l * l;
+        }
+        if (!(d > 0)) {
+            return 0;
+        }
+        d = sqrt(d);
+        band[j] = d;
+        for (size_t r = 1; r <= bw && j + r < n; r++) {
+            size_t i = j + r;
+            double v = band[r * n + i];
+            size_t kmax2 = j < (bw - r) ? j : (bw - r);
+            for (size_t k = 1; k <= kmax2; k++) {
+                v -= band[(r + k) * n + i] * band[k * n + j];
+            }
+            band[r * n + i] = v / d;
+        }
+    }
+    for (size_t i = 0; i < n; i++) {
+        double v = b[i];
+        size_t kmax = i < bw ? i : bw;
+        for (size_t k = 1; k <= kmax; k++) {
+            v -= band[k * n + i] * b[i - k];
+        }
+        b[i] = v / band[i];
+    }
+    for (size_t ii = n; ii-- > 0;) {
+        double v = b[ii];
+        for (size_t k = 1; k <= bw && ii + k < n; k++) {
+            v -= band[k * n + ii + k] * b[ii + k];
+        }
+        b[ii] = v / band[ii];
+    }
+    return 1;
+}
+
+/* Last assembled system, kept for the tests. */
+static double *nl_last_band, *nl_last_f;
+static size_t nl_last_n, nl_last_bw;
+
+/* solve_diffusion_part (g_nonlocal.c:356-592).  Nodes with m > TOL are the
+ * unknowns, numbered in ascending node order; K and f are summed in particle
+ * order as cs_entry + cs_dupl sum them.  Entries whose node carries no mass
+ * (the reference indexes past its map there) are skipped. */
+static int g_nonlocal_solve(oracle_job_t *oj)
+{
+    job_t *job = &oj->job;
+    size_t nn = job->num_nodes;
+    long *node_map = malloc(nn * sizeof(long));
+    size_t slda = 0;
+    for (size_t i = 0; i < nn; i++) {
+        node_map[i] = -1;
+    }
+    for (size_t i = 0; i < nn; i++) {
+        size_t i_new = (size_t)(job->node_number_override[NODAL_DOF * i + 0] - 0) / NODAL_DOF;
+        if (node_map[i_new] != -1) {
+            continue;
+        }
+        if (job->nodes[i_new].m > TOL) {
+            node_map[i_new] = (long)slda++;
+        }
+    }
+    if (slda == 0) {
+        free(node_map);
+        return -1;
+    }
+    /* bandwidth of the DOF numbering */
+    size_t bw = 0;
+    for (size_t i = 0; i < job->num_particles; i++) {
+        if (job->active[i] == 0 || job->in_element[i] == -1) continue;
+        const int *nd = job->elements[job->in_element[i]].nodes;
+        for (int a = 0; a < 4; a++) {
+            for (int b2 = 0; b2 < 4; b2++) {
+                long sa = node_map[nd[a]], sb = node_map[nd[b2]];
+                if (sa >= 0 && sb >= 0) {
+                    size_t d = (size_t)labs(sa - sb);
+                    if (d > bw) bw = d;
+                }
+            }
+        }
+    }
+    double *band = calloc((bw + 1) * slda, sizeof(double));
+    double *f = calloc(slda, sizeof(double));
+    double s[4], grad_s[4][2];
+    for (size_t i = 0; i < job->num_particles; i++) {
+        if (job->active[i] == 0) continue;
+        int p = job->in_element[i];
+        if (p == -1) continue;
+        const particle_t *pp = &job->particles[i];
+        s[0] = job->h1[i]; s[1] = job->h2[i]; s[2] = job->h3[i]; s[3] = job->h4[i];
+        const int *nd = job->elements[p].nodes;
+        for (int ei = 0; ei < 4; ei++) {
+            long sgi = node_map[nd[ei]];
+            if (sgi < 0) continue;
+            f[sgi] += (pp->v) * pp->state[5] * pp->state[10] * s[ei];
+        }
+    }
+    for (size_t i = 0; i < job->num_particles; i++) {
+        if (job->active[i] == 0) continue;
+        int p = job->in_element[i];
+        if (p == -1) continue;
+        const particle_t *pp = &job->particles[i];
+        s[0] = job->h1[i]; s[1] = job->h2[i]; s[2] = job->h3[i]; s[3] = job->h4[i];
+        grad_s[0][0] = job->b11[i]; grad_s[1][0] = job->b12[i];
+        grad_s[2][0] = job->b13[i]; grad_s[3][0] = job->b14[i];
+        grad_s[0][1] = job->b21[i]; grad_s[1][1] = job->b22[i];
+        grad_s[2][1] = job->b23[i]; grad_s[3][1] = job->b24[i];
+        const int *nd = job->elements[p].nodes;
+        for (int ei = 0; ei < 4; ei++) {
+            for (int ej = 0; ej < 4; ej++) {
+                long sgi = node_map[nd[ei]], sgj = node_map[nd[ej]];
+                if (sgi < 0 || sgj < 0 || sgj > sgi) continue;   /* lower band */
+                band[(size_t)(sgi - sgj) * slda + (size_t)sgi] +=
+                    pp->v * (pp->state[10] * s[ei] * s[ej]
+                             + (grad_s[ei][0] * grad_s[ej][0] + grad_s[ei][1] * grad_s[ej][1]));
+            }
+        }
+    }
+    free(nl_last_band);
+    free(nl_last_f);
+    nl_last_band = malloc((bw + 1) * slda * sizeof(double));
+    nl_last_f = malloc(slda * sizeof(double));
+    memcpy(nl_last_band, band, (bw + 1) * slda * sizeof(double));
+    memcpy(nl_last_f, f, slda * sizeof(double));
+    nl_last_n = slda;
+    nl_last_bw = bw;
+    int ok = oracle_band_cholesky_solve(band, slda, bw, f);
+    if (ok) {
+        for (size_t i = 0; i < job->num_particles; i++) {
+            if (job->active[i] == 0) continue;
+            int p = job->in_element[i];
+            if (p == -1) continue;
+            particle_t *pp = &job->particles[i];
+            s[0] = job->h1[i]; s[1] = job->h2[i]; s[2] = job->h3[i]; s[3] = job->h4[i];
+            const int *nd = job->elements[p].nodes;
+            double gf = 0;
+            for (int ei = 0; ei < 4; ei++) {
+                long sgi = node_map[nd[ei]];
+                gf += (sgi >= 0 ? f[sgi] : 0.0) * s[ei];
+            }
+            pp->state[3] = gf;
+        }
+    }
+    free(band);
+    free(f);
+    free(node_map);
+    return ok ? 0 : -1;
+}
+
+/* calculate_stress (g_nonlocal.c:103-301): every particle. */
+static void g_nonlocal_stress(oracle_job_t *oj)
+{
+    job_t *job = &oj->job;
+    const oracle_params_t *q = &oj->prm;
+    const double EMOD = q->nl_E, NUMOD = q->nl_nu, MU_S = q->nl_mu_s;
+    const double c = q->nl_c;
+    g_nonlocal_bulk(oj);
+    if (g_nonlocal_solve(oj) != 0) {
+        fprintf(stderr, "oracle: nonlocal solve failed\n");
+    }
+    for (size_t i = 0; i < job->num_particles; i++) {
+        particle_t *pp = &job->particles[i];
+        const double gf = pp->state[3];
+        double dsjxx = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->exx_t) + NUMOD * (pp->eyy_t));
+        double dsjxy = job->dt * (EMOD / (2 * (1 + NUMOD))) * (pp->exy_t);
+        double dsjyy = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->eyy_t) + NUMOD * (pp->exx_t));
+        dsjxx -= 2 * job->dt * pp->wxy_t * pp->sxy;
+        dsjxy += job->dt * pp->wxy_t * (pp->sxx - pp->syy);
+        dsjyy += 2 * job->dt * pp->wxy_t * pp->sxy;
+        double sxx_tr = pp->sxx + dsjxx;
+        double syy_tr = pp->syy + dsjyy;
+        double p_tr = -0.5 * (sxx_tr + syy_tr);
+        if (p_tr < c) {
+            pp->sxx = 0.5 * c / MU_S;
+            pp->sxy = 0;
+            pp->syy = 0.5 * c / MU_S;
+            continue;
+        }
+        /* explicit constitutive update (g_nonlocal.c:232-296) */
+        double p_t = -0.5 * (pp->sxx + pp->syy);
+        double sxx_t0 = pp->sxx + p_t;
+        double sxy_t0 = pp->sxy;
+        double syy_t0 = pp->syy + p_t;
+        double tau_t = sqrt(0.5 * (sxx_t0 * sxx_t0 + 2 * sxy_t0 * sxy_t0 + syy_t0 * syy_t0));
+        double Npxx = sqrt(0.5) * sxx_t0 / tau_t;
+        double Npxy = sqrt(0.5) * sxy_t0 / tau_t;
+        double Npyy = sqrt(0.5) * syy_t0 / tau_t;
+        double dxx0 = 0.5 * (pp->exx_t - pp->eyy_t);
+        double dxy0 = pp->exy_t;
+        double dyy0 = 0.5 * (pp->eyy_t - pp->exx_t);
+        double gammadot = sqrt(2 * (dxx0 * dxx0 + 2 * dxy0 * dxy0 + dyy0 * dyy0));
+        double nup_tau, dpxx, dpxy, dpyy;
+        if (p_t > c && gf > 0) {
+            nup_tau = gf * tau_t / p_t;
+            if (nup_tau < 0) nup_tau = 0;
+            if (nup_tau > gammadot) nup_tau = gammadot;
+            dpxx = sqrt(0.5) * (nup_tau) * Npxx;
+            dpxy = sqrt(0.5) * (nup_tau) * Npxy;
+            dpyy = sqrt(0.5) * (nup_tau) * Npyy;
+        } else {
+            nup_tau = 0;
+            dpxx = 0;
+            dpxy = 0;
+            dpyy = 0;
+        }
+        if (job->t == 0) {
+            nup_tau = 0;
+            dpxx = 0;
+            dpxy = 0;
+            dpyy = 0;
+        }
+        dsjxx = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->exx_t - dpxx) + NUMOD * (pp->eyy_t - dpyy));
+        dsjxy = job->dt * (EMOD / (2 * (1 + NUMOD))) * (pp->exy_t - dpxy);
+        dsjyy = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->eyy_t - dpyy) + NUMOD * (pp->exx_t - dpxx));
+        dsjxx -= 2 * job->dt * pp->wxy_t * pp->sxy;
+        dsjxy += job->dt * pp->wxy_t * (pp->sxx - pp->syy);
+        dsjyy += 2 * job->dt * pp->wxy_t * pp->sxy;
+        pp->state[9] += nup_tau * job->dt;
+        pp->sxx += dsjxx;
+        pp->sxy += dsjxy;
+        pp->syy += dsjyy;
+    }
+}
+
+/* The last assembled nonlocal system (lower band of K, and f). */
+size_t oracle_nonlocal_system(double *band, double *f, size_t *bw)
+{
+    if (bw) *bw = nl_last_bw;
+    if (band && nl_last_band) memcpy(band, nl_last_band, (nl_last_bw + 1) * nl_last_n * sizeof(double));
+    if (f && nl_last_f) memcpy(f, nl_last_f, nl_last_n * sizeof(double));
+    return nl_last_n;
+}
+
+/* ------------------------------------------------------------------------ */
 /* bcsrc/box.c and bcsrc/square_silo.c                                      */
 /* ------------------------------------------------------------------------ */
 static int box_validate(oracle_job_t *oj)
@@ -558,6 +840,13 @@ static void bc_time_varying(oracle_job_t *oj)
         box_time_varying(oj);
     } else if (oj->prm.bc_kind == 2) {
         silo_time_varying(oj);
+    } else {
+        /* a plugin that fixes nothing: no masks, identity node numbering */
+        job_t *job = &oj->job;
+        memset(job->u_dirichlet_mask, 0, sizeof(int) * NODAL_DOF * job->num_nodes);
+        for (size_t i = 0; i < NODAL_DOF * job->num_nodes; i++) {
+            job->node_number_override[i] = (int)i;
+        }
     }
 }
 
@@ -1041,7 +1330,16 @@ static void explicit_mpm_step_usl_threaded(threadtask_t *task)
     move_particles_explicit_usl_split(job, p_start, p_stop);
     calculate_strainrate_split(job, p_start, p_stop);
     update_particle_densities_split(job, p_start, p_stop);
-    g_local_mu2_stress_threaded(task);
+    if (oj->prm.material_kind == 1) {
+        /* g_nonlocal runs on one thread (g_nonlocal_mu2.c:113-123); it reads
+         * every particle's strain rate, so the others must be done */
+        pthread_barrier_wait(job->serialize_barrier);
+        if (task->id == 0) {
+            g_nonlocal_stress(oj);
+        }
+    } else {
+        g_local_mu2_stress_threaded(task);
+    }
 }
 
 /* mpm_run_until (src/main.c:934-982) with an exact step count in place of
@@ -1079,6 +1377,16 @@ void oracle_default_params(oracle_params_t *q)
     q->rho_crit = 1485.0f;
     q->g_mag = 9.8;
     q->ramp_time = 0.1;
+    q->material_kind = 0;
+    q->nl_E = 1e8;
+    q->nl_nu = 0.3;
+    q->nl_mu_s = 0.3819;
+    q->nl_A = 0.48;
+    q->nl_B = (0.9377 * 20);
+    q->nl_d = (0.01 * 5);
+    q->nl_rho = 2450;
+    q->nl_p_cap = 1e-2;
+    q->nl_c = 1e-2;
 }
 
 size_t oracle_params_size(void)
@@ -1156,6 +1464,12 @@ void *oracle_create(const oracle_params_t *prm, size_t np,
     find_filled_elements(job);
     initial_loads(job);
     g_local_mu2_init(oj);
+    if (prm->material_kind == 1) {
+        /* g_nonlocal.c material_init (63-93): all state zero */
+        for (size_t i = 0; i < job->num_particles; i++) {
+            memset(job->particles[i].state, 0, sizeof job->particles[i].state);
+        }
+    }
     if (prm->log_crossings) {
         job->output.log_fd = fopen(prm->log_path ? prm->log_path : "/dev/null", "w");
     }
@@ -1196,7 +1510,7 @@ enum {
     OF_X, OF_Y, OF_VX, OF_VY, OF_M, OF_V, OF_TXX, OF_TXY, OF_TYY, OF_TZZ,
     OF_GAMMAP, OF_GAMMADOTP, OF_UX, OF_UY, OF_EXX, OF_EXY, OF_EYY, OF_WXY,
     OF_AX, OF_AY, OF_XL, OF_YL, OF_SXX, OF_SXY, OF_SYY, OF_TXZ, OF_TYX,
-    OF_TYZ, OF_TZX, OF_TZY, OF_BY, OF_NFIELDS
+    OF_TYZ, OF_TZX, OF_TZY, OF_BY, OF_GF, OF_GF_BULK, OF_XISQ_INV, OF_NFIELDS
 };
 
 int oracle_num_particle_fields(void)
@@ -1215,7 +1529,8 @@ void oracle_get_particles(void *handle, double *out, int *active, int *in_elemen
             p->x, p->y, p->x_t, p->y_t, p->m, p->v, p->T[XX], p->T[XY], p->T[YY],
             p->T[ZZ], p->state[9], p->state[10], p->ux, p->uy, p->exx_t, p->exy_t,
             p->eyy_t, p->wxy_t, p->x_tt, p->y_tt, p->xl, p->yl, p->sxx, p->sxy,
-            p->syy, p->T[XZ], p->T[YX], p->T[YZ], p->T[ZX], p->T[ZY], p->by };
+            p->syy, p->T[XZ], p->T[YX], p->T[YZ], p->T[ZX], p->T[ZY], p->by,
+            p->state[3], p->state[5], p->state[10] };
         for (int k = 0; k < OF_NFIELDS; k++) {
             out[(size_t)k * np + i] = f[k];
         }

@@ -1,0 +1,162 @@
+"""g_nonlocal (materialsrc/g_nonlocal.c): the oracle's fluidity system and
+solve are pinned by mathematics, then the GPU path (assembly, conjugate
+gradients, stress update) is checked against the oracle.
+
+CPU pins of the oracle:
+* K and f are the particle quadrature of the weak form of
+  xi^-2 g - lap g = xi^-2 g_loc: for random nodal vectors u,
+  u^T K u = sum_p v_p (xi_p^-2 u_h(x_p)^2 + |grad u_h(x_p)|^2) and
+  f . u = sum_p v_p g_loc,p xi_p^-2 u_h(x_p), with u_h the bilinear
+  interpolant evaluated independently from the particle positions;
+* K is symmetric positive definite and the oracle's banded Cholesky solve
+  matches scipy.sparse.linalg.spsolve;
+* with a uniform stress state the nonlocal fluidity equals the local one
+  everywhere (K 1 = f / g_loc by the partition of unity).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import pympm
+import pyoracle
+import workloads
+
+
+def _dense_from_band(band, bw):
+    n = band.shape[1]
+    K = np.zeros((n, n))
+    for k in range(bw + 1):
+        for i in range(k, n):
+            K[i, i - k] = band[k, i]
+            K[i - k, i] = band[k, i]
+    return K
+
+
+def _one_step(w):
+    o = pyoracle.from_workload(w, num_threads=2)
+    before = w["particles"]
+    o.run(1)
+    return o, before
+
+
+def test_nonlocal_system_is_the_weak_form():
+    w = workloads.heap(cells=16, seed=5)
+    o, p0 = _one_step(w)
+    band, f, bw = o.nonlocal_system()
+    K = _dense_from_band(band, bw)
+    p = o.particles()
+    N, h = w["N"], w["h"]
+    # unknowns: nodes with mass, numbered in node order (from the oracle nodes)
+    nodes = o.nodes()
+    dof = np.flatnonzero(nodes["m"] > 1e-10)
+    assert len(dof) == len(f)
+    idx = -np.ones(N * N, dtype=int)
+    idx[dof] = np.arange(len(dof))
+    # step-start positions (the solve uses the step's own shape functions)
+    x0, y0 = p0["x"], p0["y"]
+    cx, cy = np.floor(x0 / h).astype(int), np.floor(y0 / h).astype(int)
+    xi, eta = x0 / h - cx, y0 / h - cy
+    act = p["active"] == 1
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        u = rng.normal(size=len(dof))
+        un = np.zeros(N * N)
+        un[dof] = u
+        n00 = cy * N + cx
+        c = [un[n00], un[n00 + 1], un[n00 + N + 1], un[n00 + N]]
+        uh = (c[0] * (1 - xi) * (1 - eta) + c[1] * xi * (1 - eta) + c[2] * xi * eta
+              + c[3] * (1 - xi) * eta)
+        ux = ((c[1] - c[0]) * (1 - eta) + (c[2] - c[3]) * eta) / h
+        uy = ((c[3] - c[0]) * (1 - xi) + (c[2] - c[1]) * xi) / h
+        v, xs, gl = p["v"][act], p["xisq_inv"][act], p["gf_bulk"][act]
+        quad = np.sum(v * (xs * uh[act] ** 2 + ux[act] ** 2 + uy[act] ** 2))
+        assert u @ K @ u == pytest.approx(quad, rel=1e-12)
+        assert f @ u == pytest.approx(np.sum(v * gl * xs * uh[act]), rel=1e-11, abs=1e-14)
+    ev = np.linalg.eigvalsh(K)
+    assert ev.min() > 0
+
+
+def test_band_cholesky_matches_scipy():
+    w = workloads.heap(cells=32, seed=6)
+    o, _ = _one_step(w)
+    band, f, bw = o.nonlocal_system()
+    K = sp.csc_matrix(_dense_from_band(band, bw))
+    ref = spl.spsolve(K, f)
+    lib = pyoracle.load()
+    b2, f2 = np.ascontiguousarray(band.copy()), f.copy()
+    assert lib.oracle_band_cholesky_solve(pyoracle._dptr(b2), len(f), bw, pyoracle._dptr(f2)) == 1
+    assert np.allclose(f2, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+
+
+def _uniform_block():
+    parts = workloads.block(6, 6, 16, seed=9, cell_x0=5, cell_y0=5)
+    parts["sxx"][:] = -50.0
+    parts["syy"][:] = -250.0
+    parts["sxy"][:] = 0.0
+    return dict(name="uniform_nonlocal", N=17, h=1 / 16, dt=1e-6, steps=1,
+                material=workloads.REFERENCE_MATERIAL,
+                nonlocal_material=workloads.REFERENCE_NONLOCAL, g_mag=0.0, ramp_time=0.1,
+                bc_kind=0, bc_fp64=(), bc_int=(), particles=parts)
+
+
+def test_uniform_state_gives_uniform_fluidity():
+    w = _uniform_block()
+    o, _ = _one_step(w)
+    p = o.particles()
+    assert np.all(p["gf_bulk"] > 0)
+    assert np.allclose(p["gf"], p["gf_bulk"][0], rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# GPU
+
+def _gpu_nonlocal(w, steps, tol=1e-14):
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w.get("bc_kind", 0), bc_fp64=w.get("bc_fp64", ()),
+                            bc_int=w.get("bc_int", ()), g_mag=w["g_mag"],
+                            ramp_time=w["ramp_time"], nonlocal_material=w["nonlocal_material"],
+                            cg_rel_tol=tol, cg_max_iter=50000)
+    sim = pympm.Sim(cfg, w["particles"])
+    sim.step(steps)
+    sim.sync()
+    return sim
+
+
+def _rel(a, b):
+    scale = np.abs(b).max()
+    return np.abs(a - b).max() / scale if scale > 0 else np.abs(a - b).max()
+
+
+@pytest.mark.gpu
+def test_gpu_uniform_state_gives_uniform_fluidity():
+    w = _uniform_block()
+    print("case nonlocal uniform", flush=True)
+    sim = _gpu_nonlocal(w, 1)
+    gf, gfb = sim.get("gf"), sim.get("gf_bulk")
+    assert np.all(gfb > 0)
+    assert np.allclose(gf, gfb[0], rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("steps,tol", [(1, 1e-12), (100, 1e-8)])
+def test_gpu_nonlocal_heap_matches_oracle(steps, tol):
+    w = workloads.heap(cells=32, seed=6)
+    print(f"case nonlocal heap_32 steps={steps}", flush=True)
+    sim = _gpu_nonlocal(w, steps)
+    it, res = sim.solver_stats()
+    assert res <= 1e-13
+    o = pyoracle.from_workload(w, num_threads=4)
+    o.run(steps)
+    r = o.particles()
+    pairs = {"x": "x", "y": "y", "vx": "vx", "vy": "vy", "v": "v", "txx": "sxx",
+             "txy": "sxy", "tyy": "syy", "gf": "gf", "gf_bulk": "gf_bulk",
+             "xisq_inv": "xisq_inv", "gammap": "gammap"}
+    for g, k in pairs.items():
+        err = _rel(sim.get(g), r[k])
+        # the fluidity fields carry the solver's own error (conjugate
+        # gradients to |r| <= 1e-14 |f| against a direct solve)
+        bar = max(tol, 1e-9) if g in ("gf", "gf_bulk", "xisq_inv") else tol
+        print(f"  {g}: rel err {err:.3e} (bar {bar:g})", flush=True)
+        assert err <= bar, (g, err)
+    assert np.array_equal(sim.active(), r["active"])
