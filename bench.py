@@ -143,6 +143,125 @@ def bench_heap(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_ensemble(args):
+    """256 independent column collapses of uneven size (33^2 to 181^2
+    nodes, 2,048 to 64,800 particles, 4,661,824 in all; per-member mu_s and
+    E) stepped together by shared launches replayed from a HIP graph, after
+    the bin-capacity autotuner has picked its capacity.  Beside it, the same
+    members as 256 single-job contexts (two launches per member per step,
+    each context on its own stream)."""
+    ens = workloads.ensemble()
+    counts = [len(w["particles"]["x"]) for w in ens]
+    nodes = [w["N"] * w["N"] for w in ens]
+    log(f"bench: ensemble members={len(ens)} particles={sum(counts)} warmup={args.warmup} "
+        f"steps={args.steps}")
+    cfgs = [pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                              bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
+                              g_mag=w["g_mag"], ramp_time=w["ramp_time"]) for w in ens]
+    b = pympm.Batch(cfgs, [w["particles"] for w in ens])
+    b.step(args.warmup)
+    cap, trial = b.autotune(trial_steps=4)
+    log(f"bench: autotuned bin capacity {cap} (ms/step for 16/32/64: {trial.tolist()})")
+    nact0 = sum(int(b.active(k).sum()) for k in range(len(ens)))
+    t0 = time.perf_counter()
+    ms = b.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    nact1 = sum(int(b.active(k).sum()) for k in range(len(ens)))
+    b.close()
+    pstep = (nact0 + nact1) / 2.0 * args.steps
+    np_sum, nn_sum = sum(counts), sum(nodes)
+    p2g_bytes = P2G_BYTES_PER_PARTICLE * np_sum + P2G_BYTES_PER_NODE * nn_sum
+    g2p_bytes = G2P_BYTES_PER_PARTICLE * np_sum + G2P_BYTES_PER_NODE * nn_sum
+    kernels = {"k_p2g_grid": (ms[1], p2g_bytes), "k_g2p_stress": (ms[2], g2p_bytes)}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    dom_ms, dom_bytes = kernels[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = p2g_bytes + g2p_bytes
+    step_ms = ms[0] / args.steps
+
+    # the same members as single-job contexts, stepped round-robin
+    plain_steps = max(2, min(args.steps, 50))
+    sims = [pympm.Sim(c, w["particles"]) for c, w in zip(cfgs, ens)]
+    for s in sims:
+        s.step(args.warmup)
+    for s in sims:
+        s.sync()
+    t1 = time.perf_counter()
+    for _ in range(plain_steps):
+        for s in sims:
+            s.step(1)
+    for s in sims:
+        s.sync()
+    plain_wall = time.perf_counter() - t1
+    plain_act = sum(int(s.active().sum()) for s in sims)
+    for s in sims:
+        s.close()
+    plain_value = plain_act * plain_steps / plain_wall
+
+    result = {
+        "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
+        "value": pstep / (ms[0] * 1e-3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "ms_per_step_wall": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter per member)",
+        "config": {"workload": "ensemble of 256 column collapses, 33^2-181^2 nodes, "
+                               f"{np_sum} particles, mu_s~U[0.3,0.5], E~logU[5e5,5e6], "
+                               "dt=1e-5",
+                   "members": len(ens), "particles": np_sum, "nodes": nn_sum,
+                   "bin_capacity": cap, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom, "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
+        "kernels_ms": {k: v[0] for k, v in kernels.items()},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved_GBs": step_bytes / (step_ms * 1e-3) / 1e9,
+                          "frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "autotune_ms_per_step": {"16": trial[0], "32": trial[1], "64": trial[2]},
+        "plain": {"value": plain_value, "steps": plain_steps,
+                  "how": "256 single-job contexts, one stream each, stepped round-robin"},
+        "batched_vs_plain": (pstep / (ms[0] * 1e-3)) / plain_value,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle restatement on a sample of members)")
+        result["cpu_baseline"] = cpu_baseline_members(ens[::32], args.cpu_budget)
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline_members(sample, budget_s):
+    """The CPU restatement over a sample of ensemble members, each stepped
+    for an equal share of `budget_s` on all host cores."""
+    import pyoracle
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    variant = "v4" if " avx512f" in flags else "v3"
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    psteps, el_sum, steps_sum = 0, 0.0, 0
+    for w in sample:
+        o = pyoracle.from_workload(w, variant=variant, num_threads=threads)
+        o.run(1)
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            o.run(1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s / len(sample) or steps >= 2000:
+                break
+        psteps += int(o.particles()["active"].sum()) * steps
+        el_sum += el
+        steps_sum += steps
+        o.close()
+    return {"value": psteps / el_sum, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"{len(sample)} members (every 32nd: "
+                       f"{[len(w['particles']['x']) for w in sample]} particles), "
+                       f"{steps_sum} steps in all ({el_sum:.1f} s), oracle/usl_oracle.c "
+                       f"x86-64-{variant} build, {threads} pthreads")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,9 +269,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["silo", "heap"], default="silo",
+    ap.add_argument("--workload", choices=["silo", "heap", "ensemble"], default="silo",
                     help="silo: the plain USL step (default, the headline number); "
-                         "heap: the g_nonlocal fluidity heap")
+                         "heap: the g_nonlocal fluidity heap; "
+                         "ensemble: 256 uneven members stepped as one batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the USL step is built and timed on a single "
@@ -162,6 +282,9 @@ def main():
 
     if args.workload == "heap":
         bench_heap(args)
+        return
+    if args.workload == "ensemble":
+        bench_ensemble(args)
         return
     w = workloads.silo()
     npart = len(w["particles"]["x"])

@@ -216,6 +216,50 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx);
 
 const char *mpm_hip_status_string(int status);
 
+/* ---------------------------------------------------------------------
+ * Ensembles.  A batch holds many independent jobs (members), each with its
+ * own grid size, particles, g_local_mu2 constants, boundary plugin (none,
+ * box or square_silo), dt, clock and gravity ramp, and steps all of them
+ * with one launch per kernel: P2G workgroups and particle workgroups are
+ * dealt over the members by lookup tables.  Every member's trajectory is
+ * bitwise the one mpm_hip_create + mpm_hip_step gives that job alone
+ * (explicit_mpm_step_usl_threaded, libmpm/process_usl.c:465-592, per
+ * member).  The reference has no batched entry point: it runs one job per
+ * process (plugin state is static, materialsrc/g_local_mu2.c:66-69).
+ * Steps are replayed from a captured HIP graph.
+ * ------------------------------------------------------------------- */
+typedef struct mpm_hip_batch mpm_hip_batch;
+
+/* `cfgs[k]` and `num_particles[k]` describe member k.  Members may not use
+ * MPM_BC_MASK, MPM_HIP_HOST_MATERIAL or g_nonlocal; bin_capacity (16, 32 or
+ * 64; 0 = 32) and device are taken from cfgs[0]. */
+int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles,
+                         int num_members, mpm_hip_batch **out);
+/* As mpm_hip_set_particles, for member k. */
+int mpm_hip_batch_set_particles(mpm_hip_batch *b, int member, const double *m,
+                                const double *v, const double *x, const double *y,
+                                const double *vx, const double *vy,
+                                const double *sxx, const double *sxy,
+                                const double *syy);
+int mpm_hip_batch_get_field(mpm_hip_batch *b, int member, int field, double *host);
+int mpm_hip_batch_get_active(mpm_hip_batch *b, int member, int *host);
+int mpm_hip_batch_get_time(mpm_hip_batch *b, int member, double *t, double *by);
+/* Enqueues n steps of every member; returns without waiting. */
+int mpm_hip_batch_step(mpm_hip_batch *b, int n);
+int mpm_hip_batch_sync(mpm_hip_batch *b);
+/* Runs n steps timed with HIP events on the batch's stream: ms[0] = span
+ * of all n steps; ms[1], ms[2] = P2G+grid and G2P+stress kernel durations
+ * of the last step, which is launched outside the graph with an event
+ * between its two kernels. */
+int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms);
+/* Picks the fastest bin capacity among 16, 32 and 64 that holds every
+ * cell: runs `trial_steps` steps per candidate from the current state,
+ * restores the state after each trial, and keeps the winner.  The batch's
+ * trajectory is unchanged by the choice.  *chosen = the capacity kept,
+ * ms[3] = trial time per step for 16, 32, 64 (negative: infeasible). */
+int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, double *ms);
+int mpm_hip_batch_destroy(mpm_hip_batch *b);
+
 /* Test hooks: the library's exp() on the host, and device evaluation of
  * exp, division and square root over n inputs (op 0: exp(a), 1: a / b,
  * 2: sqrt(a)) to check bitwise agreement with the host. */

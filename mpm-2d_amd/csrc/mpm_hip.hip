@@ -108,6 +108,25 @@ struct alignas(64) NodeRec {
     double m, mx, my, fx, fy, vx, vy, pad;
 };
 
+// One member of an ensemble stepped by shared launches: its own grid,
+// constants and slices of the concatenated particle, node and cell arrays.
+// Member particle ranges start at multiples of kG2PThreads, so a particle
+// workgroup never spans two members.
+struct MemberInfo {
+    GridParams g;
+    Material mt;
+    int node_base, cell_base, part_base, np;
+    int tile_base, tiles_x;
+};
+
+// Ensemble tables (device memory); unused by single-job launches.
+struct BatchTables {
+    const MemberInfo *mem;
+    const int *tile_member;    // P2G tile -> member
+    const int *block_member;   // particle workgroup -> member
+    DevState *st;              // per member
+};
+
 // ---------------------------------------------------------------------------
 // Point operations shared by both kernels.
 // ---------------------------------------------------------------------------
@@ -350,12 +369,24 @@ __device__ __forceinline__ double gravity(double t, double t0, double g_mag, dou
 
 // Element assignment + binning of all particles (first step, or after the
 // host rewrote positions).  init != 0 applies mpm_init's deactivation of
-// particles outside the grid (process_usl.c:236-244).
+// particles outside the grid (process_usl.c:236-244); in an ensemble, only
+// to member init - 1.
+template <bool BATCH>
 __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     Particles P, int *__restrict__ cnt, int *__restrict__ bkt, GridParams g,
-    DevState *__restrict__ st, int np, int init)
+    DevState *__restrict__ st, int np, int init, BatchTables bt)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (BATCH) {
+        const int mb = bt.block_member[blockIdx.x];
+        const MemberInfo &mi = bt.mem[mb];
+        g = mi.g;
+        np = mi.part_base + mi.np;
+        cnt += mi.cell_base;
+        bkt += (size_t)mi.cell_base * g.cap;
+        st = bt.st + mb;
+        init = init == mb + 1;
+    }
     if (p >= np || (!init && !P.active[p])) {
         return;
     }
@@ -367,12 +398,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     bin_insert(e >= 0, e, p, cnt, bkt, g.cap, &st->status);
 }
 
-template <int CAP>
+template <int CAP, bool BATCH>
 __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
-    int tiles_x, int ntiles)
+    int tiles_x, int ntiles, BatchTables bt)
 {
     // bin entries of one colour's cells held per thread between colours
     constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
@@ -394,9 +425,21 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // Consecutive tiles go to the same XCD (workgroups are dealt to the 8
     // XCDs round-robin), so neighbouring tiles share that XCD's L2.
     const int per_xcd = gridDim.x >> 3;
-    const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (tile >= ntiles) {
         return;
+    }
+    if constexpr (BATCH) {
+        const int mb = bt.tile_member[tile];
+        const MemberInfo &mi = bt.mem[mb];
+        g = mi.g;
+        tile -= mi.tile_base;
+        tiles_x = mi.tiles_x;
+        cnt_cur += mi.cell_base;
+        bkt_cur += (size_t)mi.cell_base * CAP;
+        cnt_next += mi.cell_base;
+        nodes += mi.node_base;
+        st = bt.st + mb;
     }
     const int tx0 = (tile % tiles_x) * kTileX;
     const int ty0 = (tile / tiles_x) * kTileY;
@@ -661,13 +704,27 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     nodes[(size_t)j * g.N + i] = r;
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     Particles P, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
     int *__restrict__ bkt_next, GridParams g, Material mt, DevState *__restrict__ st,
-    int np, int mode)
+    int np, int mode, BatchTables bt)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
+    int first = 0;
+    if constexpr (BATCH) {
+        const int mb = bt.block_member[blockIdx.x];
+        const MemberInfo &mi = bt.mem[mb];
+        g = mi.g;
+        mt = mi.mt;
+        first = mi.part_base;
+        np = mi.part_base + mi.np;
+        nodes += mi.node_base;
+        cnt_next += mi.cell_base;
+        bkt_next += (size_t)mi.cell_base * g.cap;
+        st = bt.st + mb;
+    }
+    if (p == first) {
         // t += dt (process_usl.c:522), then time_varying_loads for the next
         // step (src/main.c:975).  No thread of this kernel reads *st.
         const double t = st->t + g.dt;
@@ -1356,16 +1413,19 @@ static int launch_p2g(mpm_hip_ctx *ctx)
     const dim3 grid(ctx->p2g_blocks), block(kP2GThreads);
     switch (ctx->g.cap) {
     case 16:
-        k_p2g_grid<16><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
-            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        k_p2g_grid<16, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
+            ctx->ntiles, BatchTables{});
         break;
     case 32:
-        k_p2g_grid<32><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
-            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        k_p2g_grid<32, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
+            ctx->ntiles, BatchTables{});
         break;
     default:
-        k_p2g_grid<64><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur], ctx->bkt[cur],
-            ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x, ctx->ntiles);
+        k_p2g_grid<64, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
+            ctx->ntiles, BatchTables{});
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -1411,14 +1471,145 @@ static int launch_g2p(mpm_hip_ctx *ctx)
     if (ctx->cfg.flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) mode |= kModeStoreAux;
     if (ctx->cfg.flags & MPM_HIP_HOST_MATERIAL) mode |= kModeHostMaterial;
     if (ctx->nonlocal) mode |= kModeNonlocal | kModeStoreAux;
-    k_g2p_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+    k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
-        (int)ctx->np, mode);
+        (int)ctx->np, mode, BatchTables{});
     HIP_TRY(hipGetLastError());
     ctx->cur = nxt;
     if (ctx->nonlocal) {
         return launch_nonlocal(ctx, bins);
     }
+    return MPM_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: members concatenated in one set of arrays, stepped by shared
+// launches (the kernels' BATCH instantiations) replayed from a HIP graph.
+// ---------------------------------------------------------------------------
+
+constexpr int kGraphSteps = 8;   // steps per graph replay (even: bins return to
+                                 // the parity they had at capture)
+
+struct mpm_hip_batch {
+    int device;
+    int nmem;
+    int cap;
+    hipStream_t stream;
+    MemberInfo *hmem;          // host copy of the member table
+    mpm_hip_config *cfgs;
+    unsigned char *have;       // particles set, per member
+    MemberInfo *dmem;
+    int *tile_member, *block_member;
+    DevState *st;
+    double *pfield[MPM_F_EXX];
+    Particles P;
+    NodeRec *nodes;
+    int *cnt[2], *bkt[2];
+    size_t np_total, nn_total, ncells_total;
+    int ntiles, p2g_blocks, g2p_blocks;
+    int cur;
+    hipGraphExec_t exec;
+    int graph_cur;             // bin parity the graph was captured at
+};
+
+static BatchTables batch_tables(const mpm_hip_batch *b)
+{
+    BatchTables t;
+    t.mem = b->dmem;
+    t.tile_member = b->tile_member;
+    t.block_member = b->block_member;
+    t.st = b->st;
+    return t;
+}
+
+static int batch_launch_p2g(mpm_hip_batch *b, int cur)
+{
+    const int nxt = 1 - cur;
+    const dim3 grid(b->p2g_blocks), block(kP2GThreads);
+    const BatchTables t = batch_tables(b);
+    const GridParams g0{};
+    switch (b->cap) {
+    case 16:
+        k_p2g_grid<16, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+        break;
+    case 32:
+        k_p2g_grid<32, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+        break;
+    default:
+        k_p2g_grid<64, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
+static int batch_launch_g2p(mpm_hip_batch *b, int cur)
+{
+    const int nxt = 1 - cur;
+    const GridParams g0{};
+    const Material m0{};
+    k_g2p_stress<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
+        b->P, b->nodes, b->cnt[nxt], b->bkt[nxt], g0, m0, nullptr, 0, 0, batch_tables(b));
+    HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
+static int batch_eager_step(mpm_hip_batch *b)
+{
+    int rc = batch_launch_p2g(b, b->cur);
+    if (!rc) rc = batch_launch_g2p(b, b->cur);
+    if (!rc) b->cur = 1 - b->cur;
+    return rc;
+}
+
+static void batch_drop_graph(mpm_hip_batch *b)
+{
+    if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+}
+
+// Captures kGraphSteps steps starting at the current bin parity.
+static int batch_capture(mpm_hip_batch *b)
+{
+    batch_drop_graph(b);
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MPM_HIP_OK;
+    int c = b->cur;
+    for (int k = 0; k < kGraphSteps && !rc; ++k) {
+        rc = batch_launch_p2g(b, c);
+        if (!rc) rc = batch_launch_g2p(b, c);
+        c = 1 - c;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->stream, &graph);
+    if (rc || end != hipSuccess || !graph) {
+        if (graph) hipGraphDestroy(graph);
+        return rc ? rc : MPM_HIP_ERR_DEVICE;
+    }
+    const hipError_t inst = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+        b->exec = nullptr;
+        return MPM_HIP_ERR_DEVICE;
+    }
+    b->graph_cur = b->cur;
+    return MPM_HIP_OK;
+}
+
+static int batch_rebin(mpm_hip_batch *b, int init)
+{
+    HIP_TRY(hipMemsetAsync(b->cnt[0], 0, sizeof(int) * b->ncells_total, b->stream));
+    HIP_TRY(hipMemsetAsync(b->cnt[1], 0, sizeof(int) * b->ncells_total, b->stream));
+    b->cur = 0;
+    const GridParams g0{};
+    k_bin_particles<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
+        b->P, b->cnt[0], b->bkt[0], g0, nullptr, 0, init, batch_tables(b));
+    HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
 }
 
@@ -1429,9 +1620,83 @@ static int rebin(mpm_hip_ctx *ctx, int init)
     ctx->cur = 0;
     if (ctx->np) {
         const size_t nb = (ctx->np + kG2PThreads - 1) / kG2PThreads;
-        k_bin_particles<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-            ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g, ctx->st, (int)ctx->np, init);
+        k_bin_particles<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+            ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g, ctx->st, (int)ctx->np, init,
+            BatchTables{});
         HIP_TRY(hipGetLastError());
+    }
+    return MPM_HIP_OK;
+}
+
+// Grid, boundary and material constants of one job (mpm_init,
+// process_usl.c:68-280; bc_validate of the boundary plugins; material_init).
+static int params_from_config(const mpm_hip_config *cfg, int cap, GridParams &g, Material &mt)
+{
+    memset(&g, 0, sizeof g);
+    memset(&mt, 0, sizeof mt);
+    g.N = cfg->N;
+    g.Ne = cfg->N - 1;
+    g.ncells = g.Ne * g.Ne;
+    g.cap = cap;
+    g.h = cfg->h;
+    g.inv_h = 1.0 / cfg->h;
+    g.dt = cfg->dt;
+    g.g_mag = cfg->g_mag;
+    g.ramp = cfg->ramp_time;
+    g.bc_kind = cfg->bc_kind;
+    if (cfg->bc_kind == MPM_BC_BOX) {
+        // bc_validate (box.c:42-96)
+        const double x0 = cfg->bc_fp64[0], y0 = cfg->bc_fp64[1];
+        const double w = cfg->bc_fp64[2], hh = cfg->bc_fp64[3];
+        if (x0 < 0 || y0 < 0 || x0 > 1.0 || y0 > 1.0 || w < 0 || hh < 0
+            || (x0 + w) > 1.0 || (y0 + hh) > 1.0) {
+            return MPM_HIP_ERR_INVALID;
+        }
+        const size_t nm1 = (size_t)cfg->N - 1;
+        g.left_col = (int)(size_t)(x0 * nm1);
+        g.right_col = (int)(size_t)((x0 + w) * nm1);
+        g.bottom_row = (int)(size_t)(y0 * nm1);
+        g.top_row = (int)(size_t)((y0 + hh) * nm1);
+        g.left_fric = cfg->bc_int[0];
+        g.bottom_fric = cfg->bc_int[1];
+        g.right_fric = cfg->bc_int[2];
+        g.top_fric = cfg->bc_int[3];
+    } else if (cfg->bc_kind == MPM_BC_SQUARE_SILO) {
+        // bc_validate (square_silo.c:32-48)
+        if (!(cfg->bc_fp64[2] >= 0.0 && cfg->bc_fp64[2] <= 1.0)) {
+            return MPM_HIP_ERR_INVALID;
+        }
+        g.hole_radius = cfg->bc_fp64[0];
+        g.open_time = cfg->bc_fp64[1];
+        g.wall_col = (int)(((size_t)cfg->N - 1) * cfg->bc_fp64[2]);
+    } else if (cfg->bc_kind != MPM_BC_NONE && cfg->bc_kind != MPM_BC_MASK) {
+        return MPM_HIP_ERR_INVALID;
+    }
+
+    // material_init (g_local_mu2.c:139-142) and calculate_stress_threaded's
+    // lambda (g_local_mu2.c:187)
+    const double E = cfg->material.E, nu = cfg->material.nu;
+    mt.G = E / (2.0 * (1.0 + nu));
+    const double K = E / (3.0 * (1.0 - 2 * nu));
+    mt.two_G = 2.0 * mt.G;
+    mt.lambda = K - 2.0 * mt.G / 3.0;
+    mt.mu_s = cfg->material.mu_s;
+    mt.mu_2 = cfg->material.mu_2;
+    mt.I_0 = cfg->material.I_0;
+    mt.grains_d = cfg->material.grains_d;
+    mt.grains_rho = cfg->material.grains_rho;
+    mt.rho_crit = cfg->material.rho_crit;
+    mt.nl_E = cfg->nonlocal.E;
+    mt.nl_nu = cfg->nonlocal.nu;
+    mt.nl_mu_s = cfg->nonlocal.mu_s;
+    mt.nl_A = cfg->nonlocal.A;
+    mt.nl_B = cfg->nonlocal.B;
+    mt.nl_d = cfg->nonlocal.d;
+    mt.nl_rho = cfg->nonlocal.rho;
+    mt.nl_p_cap = cfg->nonlocal.p_cap;
+    mt.nl_c = cfg->nonlocal.c;
+    if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL) {
+        return MPM_HIP_ERR_INVALID;
     }
     return MPM_HIP_OK;
 }
@@ -1517,72 +1782,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     *out = nullptr;
 
     GridParams &g = ctx->g;
-    g.N = cfg->N;
-    g.Ne = cfg->N - 1;
-    g.ncells = g.Ne * g.Ne;
-    g.cap = cap;
-    g.h = cfg->h;
-    g.inv_h = 1.0 / cfg->h;
-    g.dt = cfg->dt;
-    g.g_mag = cfg->g_mag;
-    g.ramp = cfg->ramp_time;
-    g.bc_kind = cfg->bc_kind;
-    if (cfg->bc_kind == MPM_BC_BOX) {
-        // bc_validate (box.c:42-96)
-        const double x0 = cfg->bc_fp64[0], y0 = cfg->bc_fp64[1];
-        const double w = cfg->bc_fp64[2], hh = cfg->bc_fp64[3];
-        if (x0 < 0 || y0 < 0 || x0 > 1.0 || y0 > 1.0 || w < 0 || hh < 0
-            || (x0 + w) > 1.0 || (y0 + hh) > 1.0) {
-            delete ctx;
-            return MPM_HIP_ERR_INVALID;
-        }
-        const size_t nm1 = (size_t)cfg->N - 1;
-        g.left_col = (int)(size_t)(x0 * nm1);
-        g.right_col = (int)(size_t)((x0 + w) * nm1);
-        g.bottom_row = (int)(size_t)(y0 * nm1);
-        g.top_row = (int)(size_t)((y0 + hh) * nm1);
-        g.left_fric = cfg->bc_int[0];
-        g.bottom_fric = cfg->bc_int[1];
-        g.right_fric = cfg->bc_int[2];
-        g.top_fric = cfg->bc_int[3];
-    } else if (cfg->bc_kind == MPM_BC_SQUARE_SILO) {
-        // bc_validate (square_silo.c:32-48)
-        if (!(cfg->bc_fp64[2] >= 0.0 && cfg->bc_fp64[2] <= 1.0)) {
-            delete ctx;
-            return MPM_HIP_ERR_INVALID;
-        }
-        g.hole_radius = cfg->bc_fp64[0];
-        g.open_time = cfg->bc_fp64[1];
-        g.wall_col = (int)(((size_t)cfg->N - 1) * cfg->bc_fp64[2]);
-    } else if (cfg->bc_kind != MPM_BC_NONE && cfg->bc_kind != MPM_BC_MASK) {
-        delete ctx;
-        return MPM_HIP_ERR_INVALID;
-    }
-
-    // material_init (g_local_mu2.c:139-142) and calculate_stress_threaded's
-    // lambda (g_local_mu2.c:187)
-    Material &mt = ctx->mt;
-    const double E = cfg->material.E, nu = cfg->material.nu;
-    mt.G = E / (2.0 * (1.0 + nu));
-    const double K = E / (3.0 * (1.0 - 2 * nu));
-    mt.two_G = 2.0 * mt.G;
-    mt.lambda = K - 2.0 * mt.G / 3.0;
-    mt.mu_s = cfg->material.mu_s;
-    mt.mu_2 = cfg->material.mu_2;
-    mt.I_0 = cfg->material.I_0;
-    mt.grains_d = cfg->material.grains_d;
-    mt.grains_rho = cfg->material.grains_rho;
-    mt.rho_crit = cfg->material.rho_crit;
-    mt.nl_E = cfg->nonlocal.E;
-    mt.nl_nu = cfg->nonlocal.nu;
-    mt.nl_mu_s = cfg->nonlocal.mu_s;
-    mt.nl_A = cfg->nonlocal.A;
-    mt.nl_B = cfg->nonlocal.B;
-    mt.nl_d = cfg->nonlocal.d;
-    mt.nl_rho = cfg->nonlocal.rho;
-    mt.nl_p_cap = cfg->nonlocal.p_cap;
-    mt.nl_c = cfg->nonlocal.c;
-    if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL) {
+    if (params_from_config(cfg, cap, g, ctx->mt) != MPM_HIP_OK) {
         delete ctx;
         return MPM_HIP_ERR_INVALID;
     }
@@ -2005,6 +2205,479 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx)
     if (ctx->st) hipFree(ctx->st);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles,
+                         int num_members, mpm_hip_batch **out)
+{
+    if (!cfgs || !num_particles || !out || num_members <= 0) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    const int cap = cfgs[0].bin_capacity ? cfgs[0].bin_capacity : 32;
+    if (cap != 16 && cap != 32 && cap != 64) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    const int device = cfgs[0].device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        return MPM_HIP_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess
+        || strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        return MPM_HIP_ERR_NO_DEVICE;
+    }
+    mpm_hip_batch *b = new (std::nothrow) mpm_hip_batch();
+    if (!b) return MPM_HIP_ERR_ALLOC;
+    b->device = device;
+    b->nmem = num_members;
+    b->cap = cap;
+    b->hmem = new (std::nothrow) MemberInfo[num_members]();
+    b->cfgs = new (std::nothrow) mpm_hip_config[num_members];
+    b->have = new (std::nothrow) unsigned char[num_members]();
+    if (!b->hmem || !b->cfgs || !b->have) {
+        mpm_hip_batch_destroy(b);
+        return MPM_HIP_ERR_ALLOC;
+    }
+    // member table: particle ranges padded to whole workgroups
+    size_t pbase = 0, nbase = 0, cbase = 0;
+    long tbase = 0;
+    for (int k = 0; k < num_members; ++k) {
+        const mpm_hip_config *c = &cfgs[k];
+        if (c->N < 2 || !(c->h > 0.0) || !(c->dt > 0.0) || c->N > 46340
+            || num_particles[k] > (size_t)0x7fffffff || c->bc_kind == MPM_BC_MASK
+            || c->material_kind != MPM_MAT_G_LOCAL_MU2 || (c->flags & MPM_HIP_HOST_MATERIAL)
+            || (c->bin_capacity && c->bin_capacity != cap)) {
+            mpm_hip_batch_destroy(b);
+            return MPM_HIP_ERR_INVALID;
+        }
+        MemberInfo &mi = b->hmem[k];
+        if (params_from_config(c, cap, mi.g, mi.mt) != MPM_HIP_OK) {
+            mpm_hip_batch_destroy(b);
+            return MPM_HIP_ERR_INVALID;
+        }
+        b->cfgs[k] = *c;
+        const size_t npk = num_particles[k];
+        mi.np = (int)npk;
+        mi.part_base = (int)pbase;
+        mi.node_base = (int)nbase;
+        mi.cell_base = (int)cbase;
+        mi.tiles_x = (c->N + kTileX - 1) / kTileX;
+        mi.tile_base = (int)tbase;
+        const size_t padded = ((npk ? npk : 1) + kG2PThreads - 1) / kG2PThreads * kG2PThreads;
+        pbase += padded;
+        nbase += (size_t)c->N * c->N;
+        cbase += (size_t)mi.g.ncells;
+        tbase += (long)mi.tiles_x * ((c->N + kTileY - 1) / kTileY);
+        if (pbase > (size_t)0x7fffffff || nbase > (size_t)0x7fffffff
+            || cbase * (size_t)64 > (size_t)0x7fffffff || tbase > 0x7fffffff / 8) {
+            mpm_hip_batch_destroy(b);
+            return MPM_HIP_ERR_INVALID;
+        }
+    }
+    b->np_total = pbase;
+    b->nn_total = nbase;
+    b->ncells_total = cbase;
+    b->ntiles = (int)tbase;
+    b->p2g_blocks = ((b->ntiles + 7) / 8) * 8;
+    b->g2p_blocks = (int)(pbase / kG2PThreads);
+
+    if (hipSetDevice(device) != hipSuccess
+        || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        b->stream = nullptr;
+        mpm_hip_batch_destroy(b);
+        return MPM_HIP_ERR_DEVICE;
+    }
+#define BALLOC(ptr, bytes)                                                          \
+    do {                                                                            \
+        if (hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 64) != hipSuccess) {     \
+            mpm_hip_batch_destroy(b);                                               \
+            return MPM_HIP_ERR_ALLOC;                                               \
+        }                                                                           \
+    } while (0)
+    const size_t np = b->np_total;
+    for (int f = 0; f < MPM_F_EXX; ++f) {
+        BALLOC(b->pfield[f], np * sizeof(double));
+    }
+    Particles &P = b->P;
+    double **f = b->pfield;
+    P.x = f[MPM_F_X];
+    P.y = f[MPM_F_Y];
+    P.vx = f[MPM_F_VX];
+    P.vy = f[MPM_F_VY];
+    P.m = f[MPM_F_M];
+    P.v = f[MPM_F_V];
+    P.txx = f[MPM_F_TXX];
+    P.txy = f[MPM_F_TXY];
+    P.tyy = f[MPM_F_TYY];
+    P.tzz = f[MPM_F_TZZ];
+    P.gp = f[MPM_F_GAMMAP];
+    P.gdp = f[MPM_F_GAMMADOTP];
+    P.ux = f[MPM_F_UX];
+    P.uy = f[MPM_F_UY];
+    BALLOC(P.elem, np * sizeof(int));
+    BALLOC(P.active, np * sizeof(int));
+    BALLOC(b->nodes, b->nn_total * sizeof(NodeRec));
+    for (int k = 0; k < 2; ++k) {
+        BALLOC(b->cnt[k], b->ncells_total * sizeof(int));
+        BALLOC(b->bkt[k], b->ncells_total * (size_t)cap * sizeof(int));
+    }
+    BALLOC(b->dmem, (size_t)num_members * sizeof(MemberInfo));
+    BALLOC(b->st, (size_t)num_members * sizeof(DevState));
+    BALLOC(b->tile_member, (size_t)b->ntiles * sizeof(int));
+    BALLOC(b->block_member, (size_t)b->g2p_blocks * sizeof(int));
+#undef BALLOC
+
+    int *tm = (int *)malloc(sizeof(int) * (size_t)(b->ntiles ? b->ntiles : 1));
+    int *bm = (int *)malloc(sizeof(int) * (size_t)b->g2p_blocks);
+    DevState *s0 = (DevState *)calloc((size_t)num_members, sizeof(DevState));
+    if (!tm || !bm || !s0) {
+        free(tm);
+        free(bm);
+        free(s0);
+        mpm_hip_batch_destroy(b);
+        return MPM_HIP_ERR_ALLOC;
+    }
+    for (int k = 0; k < num_members; ++k) {
+        const MemberInfo &mi = b->hmem[k];
+        const int nt = mi.tiles_x * ((cfgs[k].N + kTileY - 1) / kTileY);
+        for (int t = 0; t < nt; ++t) tm[mi.tile_base + t] = k;
+        const int b0 = mi.part_base / kG2PThreads;
+        const int nbk = (int)(((mi.np ? mi.np : 1) + kG2PThreads - 1) / kG2PThreads);
+        for (int q = 0; q < nbk; ++q) bm[b0 + q] = k;
+        s0[k].t = cfgs[k].t0;
+        s0[k].t0 = cfgs[k].loads_t0;
+    }
+    hipError_t e = hipMemcpyAsync(b->tile_member, tm, sizeof(int) * (size_t)b->ntiles,
+                                  hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->block_member, bm, sizeof(int) * (size_t)b->g2p_blocks,
+                           hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->dmem, b->hmem, sizeof(MemberInfo) * (size_t)num_members,
+                           hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->st, s0, sizeof(DevState) * (size_t)num_members,
+                           hipMemcpyHostToDevice, b->stream);
+    for (int fld = 0; fld < MPM_F_EXX && e == hipSuccess; ++fld) {
+        e = hipMemsetAsync(b->pfield[fld], 0, np * sizeof(double), b->stream);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(P.active, 0, np * sizeof(int), b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(P.elem, 0xff, np * sizeof(int), b->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(b->nodes, 0, b->nn_total * sizeof(NodeRec), b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    free(tm);
+    free(bm);
+    free(s0);
+    if (e != hipSuccess) {
+        mpm_hip_batch_destroy(b);
+        return MPM_HIP_ERR_DEVICE;
+    }
+    *out = b;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_batch_set_particles(mpm_hip_batch *b, int member, const double *m,
+                                const double *v, const double *x, const double *y,
+                                const double *vx, const double *vy, const double *sxx,
+                                const double *sxy, const double *syy)
+{
+    if (!b || member < 0 || member >= b->nmem) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    const MemberInfo &mi = b->hmem[member];
+    if (mi.np && (!m || !v || !x || !y || !vx || !vy)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t bytes = (size_t)mi.np * sizeof(double);
+    const size_t off = (size_t)mi.part_base;
+    if (bytes) {
+        const double *src[] = {x, y, vx, vy, m, v, sxx, sxy, syy};
+        const int dst[] = {MPM_F_X, MPM_F_Y, MPM_F_VX, MPM_F_VY, MPM_F_M, MPM_F_V,
+                           MPM_F_TXX, MPM_F_TXY, MPM_F_TYY};
+        for (int k = 0; k < 9; ++k) {
+            if (src[k]) {
+                HIP_TRY(hipMemcpyAsync(b->pfield[dst[k]] + off, src[k], bytes,
+                                       hipMemcpyHostToDevice, b->stream));
+            } else {
+                HIP_TRY(hipMemsetAsync(b->pfield[dst[k]] + off, 0, bytes, b->stream));
+            }
+        }
+        const int zero[] = {MPM_F_TZZ, MPM_F_GAMMAP, MPM_F_GAMMADOTP, MPM_F_UX, MPM_F_UY};
+        for (int k = 0; k < 5; ++k) {
+            HIP_TRY(hipMemsetAsync(b->pfield[zero[k]] + off, 0, bytes, b->stream));
+        }
+    }
+    DevState s0;
+    memset(&s0, 0, sizeof s0);
+    s0.t = b->cfgs[member].t0;
+    s0.t0 = b->cfgs[member].loads_t0;
+    HIP_TRY(hipMemcpyAsync(b->st + member, &s0, sizeof s0, hipMemcpyHostToDevice, b->stream));
+    batch_drop_graph(b);
+    int rc = batch_rebin(b, member + 1);
+    if (rc) return rc;
+    b->have[member] = 1;
+    return mpm_hip_batch_sync(b);
+}
+
+static int batch_check(mpm_hip_batch *b, int member)
+{
+    if (!b || member < 0 || member >= b->nmem) return MPM_HIP_ERR_INVALID;
+    HIP_TRY(hipSetDevice(b->device));
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_batch_get_field(mpm_hip_batch *b, int member, int field, double *host)
+{
+    int rc = batch_check(b, member);
+    if (rc) return rc;
+    if (field < 0 || field >= MPM_F_EXX || (b->hmem[member].np && !host)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    const MemberInfo &mi = b->hmem[member];
+    if (mi.np) {
+        HIP_TRY(hipMemcpyAsync(host, b->pfield[field] + mi.part_base,
+                               (size_t)mi.np * sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream));
+    }
+    return mpm_hip_batch_sync(b);
+}
+
+int mpm_hip_batch_get_active(mpm_hip_batch *b, int member, int *host)
+{
+    int rc = batch_check(b, member);
+    if (rc) return rc;
+    const MemberInfo &mi = b->hmem[member];
+    if (mi.np && !host) return MPM_HIP_ERR_INVALID;
+    if (mi.np) {
+        HIP_TRY(hipMemcpyAsync(host, b->P.active + mi.part_base, (size_t)mi.np * sizeof(int),
+                               hipMemcpyDeviceToHost, b->stream));
+    }
+    return mpm_hip_batch_sync(b);
+}
+
+int mpm_hip_batch_get_time(mpm_hip_batch *b, int member, double *t, double *by)
+{
+    int rc = batch_check(b, member);
+    if (rc) return rc;
+    DevState s;
+    HIP_TRY(hipMemcpyAsync(&s, b->st + member, sizeof s, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (t) *t = s.t;
+    if (by) *by = s.by;
+    return MPM_HIP_OK;
+}
+
+static bool batch_ready(const mpm_hip_batch *b)
+{
+    for (int k = 0; k < b->nmem; ++k) {
+        if (!b->have[k]) return false;
+    }
+    return true;
+}
+
+// n steps: eager steps up to the graph's parity, whole graph replays, eager rest.
+static int batch_run(mpm_hip_batch *b, int n)
+{
+    int rc = MPM_HIP_OK;
+    if (n >= kGraphSteps) {
+        if (!b->exec) {
+            rc = batch_capture(b);
+            if (rc) return rc;
+        }
+        if (b->cur != b->graph_cur) {
+            rc = batch_eager_step(b);
+            if (rc) return rc;
+            --n;
+        }
+        while (n >= kGraphSteps) {
+            HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            n -= kGraphSteps;
+        }
+    }
+    for (; n > 0 && !rc; --n) {
+        rc = batch_eager_step(b);
+    }
+    return rc;
+}
+
+int mpm_hip_batch_step(mpm_hip_batch *b, int n)
+{
+    if (!b || n < 0 || !batch_ready(b)) return MPM_HIP_ERR_INVALID;
+    HIP_TRY(hipSetDevice(b->device));
+    return batch_run(b, n);
+}
+
+int mpm_hip_batch_sync(mpm_hip_batch *b)
+{
+    if (!b) return MPM_HIP_ERR_INVALID;
+    HIP_TRY(hipSetDevice(b->device));
+    DevState *s = (DevState *)malloc(sizeof(DevState) * (size_t)b->nmem);
+    if (!s) return MPM_HIP_ERR_ALLOC;
+    hipError_t e = hipMemcpyAsync(s, b->st, sizeof(DevState) * (size_t)b->nmem,
+                                  hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    int status = 0;
+    for (int k = 0; k < b->nmem && e == hipSuccess; ++k) status |= s[k].status;
+    free(s);
+    if (e != hipSuccess) {
+        fprintf(stderr, "mpm_hip: batch sync failed: %s\n", hipGetErrorString(e));
+        return MPM_HIP_ERR_DEVICE;
+    }
+    return (status & kStatusBinOverflow) ? MPM_HIP_ERR_BIN_OVERFLOW : MPM_HIP_OK;
+}
+
+int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms)
+{
+    if (!b || n <= 0 || !ms || !batch_ready(b)) return MPM_HIP_ERR_INVALID;
+    HIP_TRY(hipSetDevice(b->device));
+    hipEvent_t ev[4] = {};
+    int rc = MPM_HIP_OK;
+    for (int k = 0; k < 4; ++k) {
+        if (hipEventCreate(&ev[k]) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    }
+    if (!rc && hipEventRecord(ev[0], b->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    if (!rc) rc = batch_run(b, n - 1);
+    // the last step eagerly, with events around each kernel
+    if (!rc && hipEventRecord(ev[1], b->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    if (!rc) rc = batch_launch_p2g(b, b->cur);
+    if (!rc && hipEventRecord(ev[2], b->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    if (!rc) rc = batch_launch_g2p(b, b->cur);
+    if (!rc) b->cur = 1 - b->cur;
+    if (!rc && hipEventRecord(ev[3], b->stream) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    if (!rc && hipEventSynchronize(ev[3]) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    if (!rc) {
+        float a = 0.f;
+        hipEventElapsedTime(&a, ev[0], ev[3]);
+        ms[0] = a;
+        hipEventElapsedTime(&a, ev[1], ev[2]);
+        ms[1] = a;
+        hipEventElapsedTime(&a, ev[2], ev[3]);
+        ms[2] = a;
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (ev[k]) hipEventDestroy(ev[k]);
+    }
+    if (rc) return rc;
+    return mpm_hip_batch_sync(b);
+}
+
+// Replaces the bin arrays with ones of capacity `cap` and re-bins.
+static int batch_set_capacity(mpm_hip_batch *b, int cap)
+{
+    batch_drop_graph(b);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int k = 0; k < 2; ++k) {
+        if (b->bkt[k]) hipFree(b->bkt[k]);
+        b->bkt[k] = nullptr;
+        HIP_TRY(hipMalloc((void **)&b->bkt[k], b->ncells_total * (size_t)cap * sizeof(int)));
+    }
+    b->cap = cap;
+    for (int k = 0; k < b->nmem; ++k) b->hmem[k].g.cap = cap;
+    HIP_TRY(hipMemcpyAsync(b->dmem, b->hmem, sizeof(MemberInfo) * (size_t)b->nmem,
+                           hipMemcpyHostToDevice, b->stream));
+    return batch_rebin(b, 0);
+}
+
+int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, double *ms)
+{
+    if (!b || trial_steps <= 0 || !batch_ready(b)) return MPM_HIP_ERR_INVALID;
+    HIP_TRY(hipSetDevice(b->device));
+    int rc = mpm_hip_batch_sync(b);
+    if (rc) return rc;
+    // snapshot of the state a step reads
+    const size_t np = b->np_total;
+    double *save = nullptr;
+    int *save_i = nullptr;
+    DevState *save_st = nullptr;
+    HIP_TRY(hipMalloc((void **)&save, np * sizeof(double) * MPM_F_EXX));
+    if (hipMalloc((void **)&save_i, np * sizeof(int) * 2) != hipSuccess
+        || hipMalloc((void **)&save_st, sizeof(DevState) * (size_t)b->nmem) != hipSuccess) {
+        hipFree(save);
+        if (save_i) hipFree(save_i);
+        return MPM_HIP_ERR_ALLOC;
+    }
+    auto copy_state = [&](bool to_save) -> hipError_t {
+        hipError_t e = hipSuccess;
+        for (int f = 0; f < MPM_F_EXX && e == hipSuccess; ++f) {
+            double *a = save + (size_t)f * np, *p = b->pfield[f];
+            e = hipMemcpyAsync(to_save ? a : p, to_save ? p : a, np * sizeof(double),
+                               hipMemcpyDeviceToDevice, b->stream);
+        }
+        int *ia[2] = {b->P.active, b->P.elem};
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = hipMemcpyAsync(to_save ? save_i + k * np : ia[k], to_save ? ia[k] : save_i + k * np,
+                               np * sizeof(int), hipMemcpyDeviceToDevice, b->stream);
+        }
+        if (e == hipSuccess) {
+            e = hipMemcpyAsync(to_save ? save_st : b->st, to_save ? b->st : save_st,
+                               sizeof(DevState) * (size_t)b->nmem, hipMemcpyDeviceToDevice,
+                               b->stream);
+        }
+        return e;
+    };
+    const int caps[3] = {16, 32, 64};
+    double t[3] = {-1.0, -1.0, -1.0};
+    int best = -1;
+    rc = copy_state(true) == hipSuccess ? MPM_HIP_OK : MPM_HIP_ERR_DEVICE;
+    for (int k = 0; k < 3 && !rc; ++k) {
+        rc = batch_set_capacity(b, caps[k]);
+        if (!rc) {
+            rc = mpm_hip_batch_sync(b);   // overflow here: the cells do not fit
+            if (rc == MPM_HIP_ERR_BIN_OVERFLOW) {
+                rc = MPM_HIP_OK;
+            } else if (!rc) {
+                double tm[3];
+                rc = mpm_hip_batch_time_steps(b, trial_steps, tm);
+                if (rc == MPM_HIP_ERR_BIN_OVERFLOW) {
+                    rc = MPM_HIP_OK;
+                } else if (!rc) {
+                    t[k] = tm[0] / trial_steps;
+                    if (best < 0 || t[k] < t[best]) best = k;
+                }
+            }
+        }
+        if (!rc && copy_state(false) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
+    }
+    if (!rc) {
+        rc = batch_set_capacity(b, best >= 0 ? caps[best] : 64);
+    }
+    if (!rc) rc = mpm_hip_batch_sync(b);
+    hipFree(save);
+    hipFree(save_i);
+    hipFree(save_st);
+    if (rc) return rc;
+    if (best < 0) return MPM_HIP_ERR_BIN_OVERFLOW;
+    if (chosen) *chosen = caps[best];
+    if (ms) {
+        for (int k = 0; k < 3; ++k) ms[k] = t[k];
+    }
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_batch_destroy(mpm_hip_batch *b)
+{
+    if (!b) return MPM_HIP_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (int f = 0; f < MPM_F_EXX; ++f) {
+        if (b->pfield[f]) hipFree(b->pfield[f]);
+    }
+    void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->cnt[0], b->cnt[1], b->bkt[0],
+                    b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member};
+    for (void *a : arrs) {
+        if (a) hipFree(a);
+    }
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete[] b->hmem;
+    delete[] b->cfgs;
+    delete[] b->have;
+    delete b;
     return MPM_HIP_OK;
 }
 

@@ -96,6 +96,17 @@ def load():
     lib.mpm_hip_exp_host.argtypes = [ctypes.c_double]
     lib.mpm_hip_exp_host.restype = ctypes.c_double
     lib.mpm_hip_eval_ops.argtypes = [I, D, D, D, Z, I]
+    lib.mpm_hip_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Z), I,
+                                         ctypes.POINTER(P)]
+    lib.mpm_hip_batch_set_particles.argtypes = [P, I] + [D] * 9
+    lib.mpm_hip_batch_get_field.argtypes = [P, I, I, D]
+    lib.mpm_hip_batch_get_active.argtypes = [P, I, ctypes.POINTER(ctypes.c_int)]
+    lib.mpm_hip_batch_get_time.argtypes = [P, I, D, D]
+    lib.mpm_hip_batch_step.argtypes = [P, I]
+    lib.mpm_hip_batch_sync.argtypes = [P]
+    lib.mpm_hip_batch_time_steps.argtypes = [P, I, D]
+    lib.mpm_hip_batch_autotune.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
+    lib.mpm_hip_batch_destroy.argtypes = [P]
     _lib = lib
     return lib
 
@@ -223,6 +234,90 @@ class Sim:
         if self.ctx:
             self.lib.mpm_hip_destroy(self.ctx)
             self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """An ensemble of independent jobs stepped together on one GPU
+    (mpm_hip_batch_*): `cfgs[k]` and `particles[k]` describe member k."""
+
+    def __init__(self, cfgs, particles):
+        self.lib = load()
+        self.n = len(cfgs)
+        self.cfgs = (Config * self.n)(*cfgs)
+        self.np = [len(p["x"]) for p in particles]
+        counts = (ctypes.c_size_t * self.n)(*self.np)
+        self.b = ctypes.c_void_p()
+        _check(self.lib.mpm_hip_batch_create(self.cfgs, counts, self.n, ctypes.byref(self.b)),
+               "mpm_hip_batch_create")
+        for k, p in enumerate(particles):
+            arrs = [np.ascontiguousarray(p[f], dtype=np.float64)
+                    for f in ("m", "v", "x", "y", "vx", "vy")]
+            stress = [p.get(f) for f in ("sxx", "sxy", "syy")]
+            stress = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                      for a in stress]
+            ptrs = [_dptr(a) for a in arrs] + [None if a is None else _dptr(a) for a in stress]
+            _check(self.lib.mpm_hip_batch_set_particles(self.b, k, *ptrs),
+                   f"mpm_hip_batch_set_particles({k})")
+
+    def step(self, n=1):
+        _check(self.lib.mpm_hip_batch_step(self.b, n), "mpm_hip_batch_step")
+
+    def sync(self):
+        _check(self.lib.mpm_hip_batch_sync(self.b), "mpm_hip_batch_sync")
+
+    def get(self, member, name):
+        out = np.empty(self.np[member], dtype=np.float64)
+        _check(self.lib.mpm_hip_batch_get_field(self.b, member, FIELD_ID[name], _dptr(out)),
+               f"mpm_hip_batch_get_field({member}, {name})")
+        return out
+
+    def active(self, member):
+        out = np.empty(self.np[member], dtype=np.int32)
+        _check(self.lib.mpm_hip_batch_get_active(
+            self.b, member, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+            "mpm_hip_batch_get_active")
+        return out
+
+    def time(self, member):
+        t, by = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.mpm_hip_batch_get_time(self.b, member, ctypes.byref(t),
+                                               ctypes.byref(by)), "mpm_hip_batch_get_time")
+        return t.value, by.value
+
+    def state(self, member, names=None):
+        names = names or FIELDS[:14]
+        return {k: self.get(member, k) for k in names}
+
+    def time_steps(self, n):
+        ms = np.zeros(3, dtype=np.float64)
+        _check(self.lib.mpm_hip_batch_time_steps(self.b, n, _dptr(ms)),
+               "mpm_hip_batch_time_steps")
+        return ms
+
+    def autotune(self, trial_steps=4):
+        """(chosen bin capacity, per-step trial ms for capacities 16, 32, 64)."""
+        cap = ctypes.c_int()
+        ms = np.zeros(3, dtype=np.float64)
+        _check(self.lib.mpm_hip_batch_autotune(self.b, trial_steps, ctypes.byref(cap),
+                                               _dptr(ms)), "mpm_hip_batch_autotune")
+        return cap.value, ms
+
+    def close(self):
+        if self.b:
+            self.lib.mpm_hip_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -153,3 +153,26 @@ def heap(cells=256, seed=3, **kw):
              g_mag=9.81, ramp_time=0.1, **BOX_FRICTION, particles=parts)
     d.update(kw)
     return d
+
+
+def ensemble(members=256, seed=11, p_min=2048, p_max=65536, sides=None):
+    """An ensemble of column collapses of uneven size: member k has
+    log-uniform particle count in [p_min, p_max], a grid of n cells per side
+    with n = 4 round(sqrt(count / 2) / 4) (so the n/4 x n/2 cell block holds
+    2 n^2 particles), mu_s ~ U[0.3, 0.5] and E ~ log-U[5e5, 5e6]; box walls as
+    in the column collapse, dt = 1e-5.  `sides` overrides the cell counts."""
+    ids = np.arange(members, dtype=np.uint64)
+    if sides is None:
+        u = uniform01(seed, ids, 0)
+        counts = np.exp(np.log(p_min) + u * (np.log(p_max) - np.log(p_min)))
+        sides = np.clip(4 * np.rint(np.sqrt(counts / 2.0) / 4.0), 8, 46336).astype(int)
+    sides = [int(n) for n in sides]
+    mu_s = 0.3 + 0.2 * uniform01(seed, ids, 1)
+    E = np.exp(np.log(5e5) + uniform01(seed, ids, 2) * (np.log(5e6) - np.log(5e5)))
+    out = []
+    for k, n in enumerate(sides):
+        mat = dict(BASELINE_MATERIAL, mu_s=float(mu_s[k]), E=float(E[k]))
+        out.append(dict(name=f"member_{k}_{n}", N=n + 1, h=1.0 / n, dt=1e-5,
+                        material=mat, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
+                        particles=block(n // 4, n // 2, n, seed=seed * 1000 + k)))
+    return out
