@@ -232,6 +232,91 @@ def bench_ensemble(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_adjoint(args):
+    """The adjoint workload (129^2 nodes, 65,536 particles, 200 steps,
+    checkpoint every 20): a forward run keeping checkpoints, then the reverse
+    sweep for L = |x - x*|^2 + |y - y*|^2 against a run with the target
+    constants (mu_s = 0.42, E = 2e6).  A particle-step is one forward step
+    and its reverse; the plain forward run is timed beside it."""
+    w = workloads.adjoint_block()
+    steps = w["steps"]
+    npart = len(w["particles"]["x"])
+    log(f"bench: {w['name']} particles={npart} steps={steps} repeats={args.repeats}")
+
+    def cfg_for(mat):
+        return pympm.make_config(w["N"], w["h"], w["dt"], material=mat, bc_kind=w["bc_kind"],
+                                 bc_fp64=w["bc_fp64"], bc_int=w["bc_int"], g_mag=w["g_mag"],
+                                 ramp_time=w["ramp_time"])
+    tsim = pympm.Sim(cfg_for(workloads.ADJOINT_TARGET), w["particles"])
+    tsim.step(steps)
+    tx, ty = tsim.get("x"), tsim.get("y")
+    tsim.close()
+    plain = pympm.Sim(cfg_for(w["material"]), w["particles"])
+    plain.step(steps)      # warm-up
+    plain.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.repeats):
+        plain.set("x", w["particles"]["x"])
+        plain.step(steps)
+        plain.sync()
+    t_plain = (time.perf_counter() - t0) / args.repeats
+    plain.close()
+    adj = pympm.Adjoint(cfg_for(w["material"]), w["particles"], checkpoint_every=20)
+    adj.forward(steps)
+    adj.backward({"x": np.zeros(npart)})    # warm-up
+    t_f = t_b = 0.0
+    for _ in range(args.repeats):
+        t0 = time.perf_counter()
+        adj.forward(steps)
+        t1 = time.perf_counter()
+        x, y = adj.get("x"), adj.get("y")
+        t2 = time.perf_counter()
+        grad, _ = adj.backward({"x": 2 * (x - tx), "y": 2 * (y - ty)})
+        t3 = time.perf_counter()
+        t_f += t1 - t0
+        t_b += t3 - t2
+    t_f /= args.repeats
+    t_b /= args.repeats
+    adj.close()
+    loss = float(np.sum((x - tx) ** 2) + np.sum((y - ty) ** 2))
+    total = t_f + t_b
+    result = {
+        "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
+        "value": npart * steps / total,
+        "unit": "particle-steps/s (forward + reverse)",
+        "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": total * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter, lithostatic stress, initial slide)",
+        "config": {"workload": "adjoint block, 128x128 cells (129^2 nodes), 65,536 particles, "
+                               "200 steps, checkpoint every 20, g_local_mu2 "
+                               "(mu_s 0.38 vs target 0.42, E 1e6 vs target 2e6)",
+                   "particles": npart, "nodes": w["N"] ** 2, "parallelism": "single GPU"},
+        "forward_ms": t_f * 1e3, "reverse_ms": t_b * 1e3, "plain_forward_ms": t_plain * 1e3,
+        "adjoint_vs_plain": t_plain / total,
+        "loss": loss, "dL_dmu_s": grad["mu_s"], "dL_dE": grad["E"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle forward + numpy reverse sweep, 20 steps)")
+        import adjoint_oracle
+        nsteps = 20
+        t0 = time.perf_counter()
+        recs, fin = adjoint_oracle.record(w, nsteps, num_threads=1)
+        adjoint_oracle.adjoint(recs, {"x": fin["x"], "y": fin["y"]}, w["material"], w["N"],
+                               w["h"], w["dt"])
+        el = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": npart * nsteps / el, "unit": "particle-steps/s (forward + reverse)",
+            "cores": 1, "kind": "port",
+            "sample": (f"the first {nsteps} of the 200 steps ({el:.1f} s): oracle/usl_oracle.c "
+                       "forward on 1 thread, recorded step by step, then the numpy reverse "
+                       "sweep of oracle/adjoint_oracle.py")}
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
 def cpu_baseline_members(sample, budget_s):
     """The CPU restatement over a sample of ensemble members, each stepped
     for an equal share of `budget_s` on all host cores."""
@@ -269,10 +354,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["silo", "heap", "ensemble"], default="silo",
+    ap.add_argument("--repeats", type=int, default=3, help="adjoint workload: timed runs")
+    ap.add_argument("--workload", choices=["silo", "heap", "ensemble", "adjoint"],
+                    default="silo",
                     help="silo: the plain USL step (default, the headline number); "
                          "heap: the g_nonlocal fluidity heap; "
-                         "ensemble: 256 uneven members stepped as one batch")
+                         "ensemble: 256 uneven members stepped as one batch; "
+                         "adjoint: forward + reverse sweep of a 200-step run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the USL step is built and timed on a single "
@@ -285,6 +373,9 @@ def main():
         return
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
     w = workloads.silo()
     npart = len(w["particles"]["x"])

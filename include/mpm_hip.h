@@ -260,6 +260,47 @@ int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms);
 int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, double *ms);
 int mpm_hip_batch_destroy(mpm_hip_batch *b);
 
+/* ---------------------------------------------------------------------
+ * Adjoint runs.  Reverse-mode derivative of a run of n USL steps with the
+ * g_local_mu2 law (libmpm/process_usl.c:465-592,
+ * materialsrc/g_local_mu2.c:197-307) with respect to the initial particle
+ * state and the material constants.  The forward run keeps a checkpoint
+ * every `checkpoint_every` steps; the reverse sweep re-runs each segment
+ * from its checkpoint, then steps back through it with hand-written
+ * reverse kernels.  Discrete choices (particle cells, active flags, node
+ * mass thresholds, boundary masks, the material branch) are those of the
+ * forward run; the result is the derivative of the run on that branch.
+ * The reference has no derivative of its step.
+ * ------------------------------------------------------------------- */
+typedef struct mpm_hip_adjoint mpm_hip_adjoint;
+
+/* g_local_mu2 only (no MPM_HIP_HOST_MATERIAL, no g_nonlocal). */
+int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
+                           int checkpoint_every, mpm_hip_adjoint **out);
+/* The initial state, as mpm_hip_set_particles. */
+int mpm_hip_adjoint_set_particles(mpm_hip_adjoint *a, const double *m, const double *v,
+                                  const double *x, const double *y,
+                                  const double *vx, const double *vy,
+                                  const double *sxx, const double *sxy,
+                                  const double *syy);
+/* Replaces the material constants for the next forward run. */
+int mpm_hip_adjoint_set_material(mpm_hip_adjoint *a, const mpm_hip_material *mat);
+/* Runs n steps from the initial state, keeping the checkpoints. */
+int mpm_hip_adjoint_forward(mpm_hip_adjoint *a, int nsteps);
+/* State after the forward run (before mpm_hip_adjoint_backward). */
+int mpm_hip_adjoint_get_field(mpm_hip_adjoint *a, int field, double *host);
+int mpm_hip_adjoint_get_active(mpm_hip_adjoint *a, int *host);
+/* seed[f] (f < MPM_F_EXX; NULL array or entry = zero) is dL/d(field f of
+ * the final state).  Writes dL/d(E, nu, mu_s, mu_2, I_0, grains_d,
+ * grains_rho) to grad_params[7] and, for each non-NULL grad_init[f], dL/d
+ * (field f of the initial state); the initial Txx, Txy, Tyy entries are
+ * the derivatives with respect to the particle file's sxx, sxy, syy.  The
+ * mass entry is not formed (zero).  Consumes the forward run: call
+ * mpm_hip_adjoint_forward again before the next backward. */
+int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
+                             double *grad_params, double *const *grad_init);
+int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a);
+
 /* Test hooks: the library's exp() on the host, and device evaluation of
  * exp, division and square root over n inputs (op 0: exp(a), 1: a / b,
  * 2: sqrt(a)) to check bitwise agreement with the host. */

@@ -1303,6 +1303,420 @@ __global__ __launch_bounds__(kG2PThreads) void k_nl_stress(
     P.tyy[p] = syy + dsjyy;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (reverse mode) of the USL step with g_local_mu2.
+//
+// The step is differentiated on its recorded branch: particle cells, active
+// flags, node-mass thresholds, boundary masks and the material branch
+// (density cut-off / elastic / plastic) are held fixed.  One reverse step is
+// three kernels after the forward node state of the step has been rebuilt
+// (k_p2g_grid on the step's start state):
+//   k_adj_g2p    per particle: reverse of the stress update
+//                (g_local_mu2.c:197-307), the volume update
+//                (process_usl.c:961-970), the strain rate (764-808) and G2P
+//                (914-957); leaves in the particle adjoint the terms that do
+//                not pass through the grid, and 8 coefficients that carry the
+//                rest to the nodes.
+//   k_adj_nodes  per node: gathers those coefficients from the particles of
+//                the node's four cells (bins sorted by index, so the sums are
+//                deterministic), then the reverse of move_grid_split
+//                (596-625) and of the boundary zeroing (box.c:183-224).
+//   k_adj_p2g    per particle: reverse of the P2G scatter
+//                (process_usl.c:812-910, map.c:85-104) by a gather from the
+//                particle's four nodes.
+// ---------------------------------------------------------------------------
+
+constexpr int kAdjParams = 7;   // adjoints of G, lambda, mu_s, mu_2, I_0, d, rho_s
+constexpr int kAdjCoef = 8;     // A, B, Cx, Cy, exx~, dxy~, eyy~, dyx~ per particle
+
+struct AdjState {
+    double *lam[MPM_F_EXX];     // particle adjoint, updated in place
+    double *coef;               // [kAdjCoef][np]
+    double *nadj;               // [5][nn]: m~, px~, py~, fx~, fy~
+    double *partials;           // [kAdjParams][blocks]
+};
+
+// Reverse of g_local_mu2_update.  In: the update's inputs and the adjoints
+// of its outputs (Tb[4] for Txx, Txy, Tyy, Tzz; gpb, gdpb).  Out: adjoints of
+// the strain-rate inputs, of the input stress (written over Tb) and of the
+// constants (pb[kAdjParams], added).
+__device__ __forceinline__ void g_local_mu2_reverse(
+    double exx, double exy, double eyy, double wxy, double m, double v_new, double dt,
+    const Material &mt, double txx, double txy, double tyy, double tzz, double Tb[4],
+    double gpb, double gdpb, double &exxb, double &exyb, double &eyyb, double &wxyb,
+    double pb[kAdjParams])
+{
+    const double w = 0.5 * ((exy + wxy) - (exy - wxy));
+    const double dxx = 0.5 * (exx + exx);
+    const double dxy = 0.5 * ((exy + wxy) + (exy - wxy));
+    const double dyy = 0.5 * (eyy + eyy);
+    const double trD = (dxx + dyy) + 0.0;
+    const double a = w * txy;
+    const double ltr = mt.lambda * trD;
+    const double rxx = ((dxx * mt.two_G + ltr) + (a + a)) * dt + txx;
+    const double rxy = ((dxy * mt.two_G) + (w * tyy - w * txx)) * dt + txy;
+    const double ryy = ((dyy * mt.two_G + ltr) + -(a + a)) * dt + tyy;
+    const double rzz = ltr * dt + tzz;
+    const double c = ((rxx + ryy) + rzz) / 3.0;
+    const double d_xx = rxx - c, d_yy = ryy - c, d_zz = rzz - c;
+    const double p = c * -1.0;
+    double tau2 = d_xx * d_xx;
+    tau2 += rxy * rxy;
+    tau2 += rxy * rxy;
+    tau2 += d_yy * d_yy;
+    tau2 += d_zz * d_zz;
+    const double tau = sqrt(0.5 * tau2);
+
+    double rxxb = 0.0, rxyb = 0.0, ryyb = 0.0, rzzb = 0.0;
+    exxb = exyb = eyyb = wxyb = 0.0;
+    const bool zero = (m / v_new) < mt.rho_crit || p <= 0.0;
+    if (zero) {
+        Tb[0] = Tb[1] = Tb[2] = Tb[3] = 0.0;
+        return;
+    }
+    if (!(p > 0.0)) {
+        return;   // NaN pressure: the stress passes through unchanged
+    }
+    const double S0 = mt.mu_s * p;
+    if (tau <= S0) {
+        rxxb = Tb[0];
+        rxyb = Tb[1];
+        ryyb = Tb[2];
+        rzzb = Tb[3];
+    } else {
+        const double S2 = mt.mu_2 * p;
+        const double alpha = mt.G * mt.I_0 * dt * sqrt(p / mt.grains_rho) / mt.grains_d;
+        const double B = -(S2 + tau + alpha);
+        const double H = S2 * tau + S0 * alpha;
+        double taup;
+        if (B > 0.0) {
+            taup = (-B - sqrt(B * B - 4.0 * H)) / 2.0;
+        } else {
+            taup = (2.0 * H) / (-B + sqrt(B * B - 4.0 * H));
+        }
+        const double scale = taup / tau;
+        const double nupb = gdpb + gpb * dt;
+        const double scaleb = Tb[0] * d_xx + Tb[1] * rxy + Tb[2] * d_yy + Tb[3] * d_zz;
+        double dxxb = Tb[0] * scale, dyyb = Tb[2] * scale, dzzb = Tb[3] * scale;
+        double rxyb_p = Tb[1] * scale;
+        double Pb = -(Tb[0] + Tb[2] + Tb[3]);
+        const double Gdt = mt.G * dt;
+        double taub = nupb / Gdt;
+        double taupb = -nupb / Gdt + scaleb / tau;
+        double Gb = -nupb * (tau - taup) / (mt.G * Gdt);
+        taub -= scaleb * taup / (tau * tau);
+        const double den = 2.0 * taup + B;
+        const double Bb = -taupb * taup / den;
+        const double Hb = -taupb / den;
+        const double S2b = -Bb + Hb * tau;
+        taub += -Bb + Hb * S2;
+        const double alphab = -Bb + Hb * S0;
+        const double S0b = Hb * alpha;
+        Gb += alphab * alpha / mt.G;
+        pb[4] += alphab * alpha / mt.I_0;
+        pb[5] += -alphab * alpha / mt.grains_d;
+        pb[6] += -alphab * alpha / (2.0 * mt.grains_rho);
+        Pb += alphab * alpha / (2.0 * p);
+        pb[2] += S0b * p;
+        Pb += S0b * mt.mu_s;
+        pb[3] += S2b * p;
+        Pb += S2b * mt.mu_2;
+        pb[0] += Gb;
+        const double tau2b = taub * 0.25 / tau;
+        dxxb += 2.0 * d_xx * tau2b;
+        rxyb_p += 4.0 * rxy * tau2b;
+        dyyb += 2.0 * d_yy * tau2b;
+        dzzb += 2.0 * d_zz * tau2b;
+        const double cb = -Pb - (dxxb + dyyb + dzzb);
+        rxxb = dxxb + cb / 3.0;
+        ryyb = dyyb + cb / 3.0;
+        rzzb = dzzb + cb / 3.0;
+        rxyb = rxyb_p;
+    }
+    // trial stress
+    const double jsxxb = rxxb * dt, jsxyb = rxyb * dt, jsyyb = ryyb * dt;
+    const double ltrb = (rxxb + ryyb + rzzb) * dt;
+    pb[0] += 2.0 * (rxxb * dxx + rxyb * dxy + ryyb * dyy) * dt;
+    pb[1] += ltrb * trD;
+    const double trDb = ltrb * mt.lambda;
+    exxb = rxxb * dt * mt.two_G + trDb;
+    exyb = rxyb * dt * mt.two_G;
+    eyyb = ryyb * dt * mt.two_G + trDb;
+    wxyb = 2.0 * txy * (jsxxb - jsyyb) + jsxyb * (tyy - txx);
+    Tb[0] = rxxb - jsxyb * w;
+    Tb[1] = rxyb + 2.0 * w * (jsxxb - jsyyb);
+    Tb[2] = ryyb + jsxyb * w;
+    Tb[3] = rzzb;
+}
+
+__device__ __forceinline__ void block_sum_params(double v[kAdjParams], double *s_red)
+{
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            v[k] += __shfl_down(v[k], d);
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kAdjParams; ++k) s_red[k * nw + wv] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) {
+        double t = 0.0;
+        for (int q = 0; q < nw; ++q) t += s_red[k * nw + q];
+        v[k] = t;
+    }
+}
+
+__global__ __launch_bounds__(kG2PThreads) void k_adj_g2p(
+    Particles P, const NodeRec *__restrict__ nodes, AdjState A, GridParams g, Material mt,
+    int np)
+{
+    __shared__ double s_red[kAdjParams * (kG2PThreads / 64)];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double pb[kAdjParams] = {0, 0, 0, 0, 0, 0, 0};
+    const bool upd = p < np && P.active[p] && P.elem[p] >= 0;
+    if (p < np) {
+        A.lam[MPM_F_M][p] = 0.0;
+    }
+    if (p < np && !upd) {
+#pragma unroll
+        for (int k = 0; k < kAdjCoef; ++k) A.coef[(size_t)k * np + p] = 0.0;
+    }
+    if (upd) {
+        const int e = P.elem[p];
+        const int cc = e % g.Ne, rr = e / g.Ne;
+        const double dt = g.dt;
+        Shape sh;
+        shape_functions(P.x[p], P.y[p], cc, rr, g, sh);
+        const size_t n0 = (size_t)rr * g.N + cc;
+        const size_t nid[4] = {n0, n0 + 1, n0 + g.N + 1, n0 + g.N};
+        NodeRec nd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nd[q] = nodes[nid[q]];
+        const double exx = 1.0 * (sh.b1[0] * nd[0].vx + sh.b1[1] * nd[1].vx
+                                  + sh.b1[2] * nd[2].vx + sh.b1[3] * nd[3].vx);
+        const double eyy = 1.0 * (sh.b2[0] * nd[0].vy + sh.b2[1] * nd[1].vy
+                                  + sh.b2[2] * nd[2].vy + sh.b2[3] * nd[3].vy);
+        const double dxy = 1.0 * (sh.b2[0] * nd[0].vx + sh.b2[1] * nd[1].vx
+                                  + sh.b2[2] * nd[2].vx + sh.b2[3] * nd[3].vx);
+        const double dyx = 1.0 * (sh.b1[0] * nd[0].vy + sh.b1[1] * nd[1].vy
+                                  + sh.b1[2] * nd[2].vy + sh.b1[3] * nd[3].vy);
+        const double exy = 0.5 * (dxy + dyx);
+        const double wxy = 0.5 * (dxy - dyx);
+        const double ex = mpm_exp(dt * (exx + eyy));
+        const double v = P.v[p];
+        const double v_new = v * ex;
+        const double mp = P.m[p];
+
+        double lb[MPM_F_EXX];
+#pragma unroll
+        for (int f = 0; f < MPM_F_EXX; ++f) lb[f] = A.lam[f][p];
+        double Tb[4] = {lb[MPM_F_TXX], lb[MPM_F_TXY], lb[MPM_F_TYY], lb[MPM_F_TZZ]};
+        double exxb, exyb, eyyb, wxyb;
+        g_local_mu2_reverse(exx, exy, eyy, wxy, mp, v_new, dt, mt, P.txx[p], P.txy[p],
+                            P.tyy[p], P.tzz[p], Tb, lb[MPM_F_GAMMAP], lb[MPM_F_GAMMADOTP],
+                            exxb, exyb, eyyb, wxyb, pb);
+        // volume
+        const double vb = lb[MPM_F_V] * ex;
+        exxb += lb[MPM_F_V] * v_new * dt;
+        eyyb += lb[MPM_F_V] * v_new * dt;
+        const double dxyb = 0.5 * (exyb + wxyb);
+        const double dyxb = 0.5 * (exyb - wxyb);
+        // G2P
+        const double Ab = lb[MPM_F_VX] * dt, Bb = lb[MPM_F_VY] * dt;
+        const double Cx = (lb[MPM_F_X] + lb[MPM_F_UX]) * dt;
+        const double Cy = (lb[MPM_F_Y] + lb[MPM_F_UY]) * dt;
+        const double r2 = g.inv_h * g.inv_h;
+        double xb = lb[MPM_F_X], yb = lb[MPM_F_Y];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double Sb = 0.0;
+            if (!(nd[q].m < kTol)) {
+                Sb = (Ab * nd[q].fx + Bb * nd[q].fy + Cx * nd[q].mx + Cy * nd[q].my) / nd[q].m;
+            }
+            const double b1b = exxb * nd[q].vx + dyxb * nd[q].vy;
+            const double b2b = eyyb * nd[q].vy + dxyb * nd[q].vx;
+            const double sg = (q & 1) ? -1.0 : 1.0;
+            xb += Sb * sh.b1[q] + b2b * sg * r2;
+            yb += Sb * sh.b2[q] + b1b * sg * r2;
+        }
+        A.lam[MPM_F_X][p] = xb;
+        A.lam[MPM_F_Y][p] = yb;
+        A.lam[MPM_F_V][p] = vb;
+        A.lam[MPM_F_TXX][p] = Tb[0];
+        A.lam[MPM_F_TXY][p] = Tb[1];
+        A.lam[MPM_F_TYY][p] = Tb[2];
+        A.lam[MPM_F_TZZ][p] = Tb[3];
+        A.lam[MPM_F_GAMMADOTP][p] = 0.0;
+        const double cf[kAdjCoef] = {Ab, Bb, Cx, Cy, exxb, dxyb, eyyb, dyxb};
+#pragma unroll
+        for (int k = 0; k < kAdjCoef; ++k) A.coef[(size_t)k * np + p] = cf[k];
+    }
+    block_sum_params(pb, s_red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kAdjParams; ++k) A.partials[(size_t)k * gridDim.x + blockIdx.x] = pb[k];
+    }
+}
+
+// Sorts every cell's bin by particle index (one thread per cell).
+__global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, int *__restrict__ bkt,
+                                                   int ncells, int cap)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int n = cnt[c];
+    n = n < cap ? n : cap;
+    int *l = bkt + (size_t)c * cap;
+    for (int k = 1; k < n; ++k) {
+        const int key = l[k];
+        int r = k - 1;
+        while (r >= 0 && l[r] > key) {
+            l[r + 1] = l[r];
+            --r;
+        }
+        l[r + 1] = key;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_nodes(
+    Particles P, const int *__restrict__ cnt, const int *__restrict__ bkt,
+    const NodeRec *__restrict__ nodes, const uint8_t *__restrict__ mask, AdjState A,
+    GridParams g, const DevState *__restrict__ st, int np)
+{
+    const int nidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = g.N * g.N;
+    if (nidx >= nn) return;
+    const int i = nidx % g.N, j = nidx / g.N;
+    double Fx = 0, Fy = 0, Ux = 0, Uy = 0, Vx = 0, Vy = 0;
+    // the node's cells in colour order, with the node's corner index in each
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int pc = c & 1, pr = c >> 1;
+        const int cc = i - ((i ^ pc) & 1);
+        const int rr = j - ((j ^ pr) & 1);
+        if (cc < 0 || rr < 0 || cc >= g.Ne || rr >= g.Ne) continue;
+        const int dx = i - cc, dy = j - rr;
+        const int q = dy ? (dx ? 2 : 3) : (dx ? 1 : 0);
+        const int cell = rr * g.Ne + cc;
+        int n = cnt[cell];
+        n = n < g.cap ? n : g.cap;
+        for (int k = 0; k < n; ++k) {
+            const int p = bkt[(size_t)cell * g.cap + k];
+            Shape sh;
+            shape_functions(P.x[p], P.y[p], cc, rr, g, sh);
+            const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
+            Fx += S * A.coef[(size_t)0 * np + p];
+            Fy += S * A.coef[(size_t)1 * np + p];
+            Ux += S * A.coef[(size_t)2 * np + p];
+            Uy += S * A.coef[(size_t)3 * np + p];
+            Vx += b1 * A.coef[(size_t)4 * np + p] + b2 * A.coef[(size_t)5 * np + p];
+            Vy += b2 * A.coef[(size_t)6 * np + p] + b1 * A.coef[(size_t)7 * np + p];
+        }
+    }
+    const NodeRec nd = nodes[nidx];
+    const double m = nd.m;
+    double mb = 0.0, pxb = 0.0, pyb = 0.0, fxb = 0.0, fyb = 0.0;
+    if (!(m < kTol)) {
+        fxb = Fx / m;
+        fyb = Fy / m;
+        pxb = Ux / m;
+        pyb = Uy / m;
+        mb = -(Fx * nd.fx + Fy * nd.fy + Ux * nd.mx + Uy * nd.my) / (m * m);
+    }
+    if (m > kTol) {
+        // v = p~ / m, p~ = p + dt f
+        pxb += Vx / m;
+        pyb += Vy / m;
+        mb -= (Vx * nd.mx + Vy * nd.my) / (m * m);
+        fxb += g.dt * pxb;
+        fyb += g.dt * pyb;
+    }
+    const unsigned f = bc_fixed(g, i, j, st->t + g.dt, mask);
+    if (f & 1u) {
+        pxb = 0.0;
+        fxb = 0.0;
+    }
+    if (f & 2u) {
+        pyb = 0.0;
+        fyb = 0.0;
+    }
+    A.nadj[(size_t)0 * nn + nidx] = mb;
+    A.nadj[(size_t)1 * nn + nidx] = pxb;
+    A.nadj[(size_t)2 * nn + nidx] = pyb;
+    A.nadj[(size_t)3 * nn + nidx] = fxb;
+    A.nadj[(size_t)4 * nn + nidx] = fyb;
+}
+
+__global__ __launch_bounds__(kG2PThreads) void k_adj_p2g(
+    Particles P, AdjState A, GridParams g, const DevState *__restrict__ st, int np)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np || !P.active[p] || P.elem[p] < 0) return;
+    const int e = P.elem[p];
+    const int cc = e % g.Ne, rr = e / g.Ne;
+    Shape sh;
+    shape_functions(P.x[p], P.y[p], cc, rr, g, sh);
+    const int nn = g.N * g.N;
+    const size_t n0 = (size_t)rr * g.N + cc;
+    const size_t nid[4] = {n0, n0 + 1, n0 + g.N + 1, n0 + g.N};
+    const double m = P.m[p], v = P.v[p], vx = P.vx[p], vy = P.vy[p];
+    const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
+    const double bx = st->bx, by = st->by;
+    const double r2 = g.inv_h * g.inv_h;
+    double xb = 0, yb = 0, vxb = 0, vyb = 0, vb = 0, txxb = 0, txyb = 0, tyyb = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double mbj = A.nadj[(size_t)0 * nn + nid[q]];
+        const double pxbj = A.nadj[(size_t)1 * nn + nid[q]];
+        const double pybj = A.nadj[(size_t)2 * nn + nid[q]];
+        const double fxbj = A.nadj[(size_t)3 * nn + nid[q]];
+        const double fybj = A.nadj[(size_t)4 * nn + nid[q]];
+        const double Sb = mbj * m + pxbj * (vx * m) + pybj * (vy * m) + fxbj * (bx * m)
+                          + fybj * (by * m);
+        const double b1b = fxbj * (-sxx * v) + fybj * (-sxy * v);
+        const double b2b = fxbj * (-sxy * v) + fybj * (-syy * v);
+        const double sg = (q & 1) ? -1.0 : 1.0;
+        xb += Sb * sh.b1[q] + b2b * sg * r2;
+        yb += Sb * sh.b2[q] + b1b * sg * r2;
+        vxb += sh.s[q] * pxbj;
+        vyb += sh.s[q] * pybj;
+        txxb += sh.b1[q] * fxbj;
+        txyb += sh.b2[q] * fxbj + sh.b1[q] * fybj;
+        tyyb += sh.b2[q] * fybj;
+        vb += fxbj * (-sxx * sh.b1[q] - sxy * sh.b2[q]) + fybj * (-sxy * sh.b1[q] - syy * sh.b2[q]);
+    }
+    A.lam[MPM_F_X][p] += xb;
+    A.lam[MPM_F_Y][p] += yb;
+    A.lam[MPM_F_VX][p] += m * vxb;
+    A.lam[MPM_F_VY][p] += m * vyb;
+    A.lam[MPM_F_V][p] += vb;
+    A.lam[MPM_F_TXX][p] += -v * txxb;
+    A.lam[MPM_F_TXY][p] += -v * txyb;
+    A.lam[MPM_F_TYY][p] += -v * tyyb;
+}
+
+// acc[k] += sum over workgroups of the step's parameter partials, in order.
+__global__ __launch_bounds__(256) void k_adj_reduce(const double *__restrict__ partials,
+                                                    int nblocks, double *__restrict__ acc)
+{
+    __shared__ double s_red[4];
+    for (int k = 0; k < kAdjParams; ++k) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            v += partials[(size_t)k * nblocks + b];
+        }
+        v = block_sum(v, s_red);
+        if (threadIdx.x == 0) acc[k] += v;
+    }
+}
+
+
 __global__ void k_eval_ops(int op, const double *a, const double *b, double *out, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2678,6 +3092,293 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
     delete[] b->cfgs;
     delete[] b->have;
     delete b;
+    return MPM_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint runs: forward with checkpoints, reverse sweep segment by segment.
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct mpm_hip_adjoint {
+    mpm_hip_ctx *ctx;          // forward state and kernels
+    int K;                     // checkpoint interval
+    int nsteps;                // steps of the last forward run
+    int ncp;                   // checkpoints allocated
+    size_t np;
+    double *init;              // [MPM_F_EXX][np] state after set_particles
+    int *init_active;
+    DevState *init_st;
+    double *cp;                // [ncp][MPM_F_EXX][np]
+    int *cp_active;            // [ncp][np]
+    DevState *cp_st;           // [ncp]
+    double *seg;               // [K][MPM_F_EXX][np]
+    int *seg_active;           // [K][np]
+    DevState *seg_st;          // [K]
+    AdjState A;
+    double *lam_store;         // [MPM_F_EXX][np]
+    double *acc;               // [kAdjParams]
+    int g2p_blocks;
+    bool have_particles, have_forward;
+};
+
+static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, DevState *st)
+{
+    mpm_hip_ctx *ctx = a->ctx;
+    const size_t bytes = a->np * sizeof(double);
+    for (int k = 0; k < MPM_F_EXX; ++k) {
+        double *dev = ctx->pfield[k], *buf = f + (size_t)k * a->np;
+        if (bytes) {
+            HIP_TRY(hipMemcpyAsync(save ? buf : dev, save ? dev : buf, bytes,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    if (a->np) {
+        HIP_TRY(hipMemcpyAsync(save ? act : ctx->P.active, save ? ctx->P.active : act,
+                               a->np * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(save ? st : ctx->st, save ? ctx->st : st, sizeof(DevState),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    if (!save) {
+        return rebin(ctx, 0);
+    }
+    return MPM_HIP_OK;
+}
+
+// One reverse step for the step whose start state is loaded in the context.
+static int adj_reverse_step(mpm_hip_adjoint *a)
+{
+    mpm_hip_ctx *ctx = a->ctx;
+    int rc = launch_p2g(ctx);   // the step's node state
+    if (rc) return rc;
+    const int np = (int)a->np;
+    const int nn = ctx->g.N * ctx->g.N;
+    k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
+        ctx->P, ctx->nodes, a->A, ctx->g, ctx->mt, np);
+    k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->g.ncells, ctx->g.cap);
+    k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        ctx->P, ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->nodes, ctx->mask, a->A, ctx->g,
+        ctx->st, np);
+    k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
+        ctx->P, a->A, ctx->g, ctx->st, np);
+    k_adj_reduce<<<dim3(1), dim3(256), 0, ctx->stream>>>(a->A.partials, a->g2p_blocks, a->acc);
+    HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
+extern "C" {
+
+int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
+                           int checkpoint_every, mpm_hip_adjoint **out)
+{
+    if (!cfg || !out || checkpoint_every <= 0 || cfg->material_kind != MPM_MAT_G_LOCAL_MU2
+        || (cfg->flags & MPM_HIP_HOST_MATERIAL)) {
+        return MPM_HIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    mpm_hip_adjoint *a = new (std::nothrow) mpm_hip_adjoint();
+    if (!a) return MPM_HIP_ERR_ALLOC;
+    int rc = mpm_hip_create(cfg, num_particles, &a->ctx);
+    if (rc) {
+        delete a;
+        return rc;
+    }
+    a->K = checkpoint_every;
+    a->np = num_particles;
+    const size_t np = num_particles ? num_particles : 1;
+    const size_t nn = (size_t)cfg->N * cfg->N;
+    a->g2p_blocks = (int)((np + kG2PThreads - 1) / kG2PThreads);
+#define AALLOC(ptr, bytes)                                                          \
+    do {                                                                            \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                    \
+            mpm_hip_adjoint_destroy(a);                                             \
+            return MPM_HIP_ERR_ALLOC;                                               \
+        }                                                                           \
+    } while (0)
+    AALLOC(a->init, np * sizeof(double) * MPM_F_EXX);
+    AALLOC(a->init_active, np * sizeof(int));
+    AALLOC(a->init_st, sizeof(DevState));
+    AALLOC(a->seg, np * sizeof(double) * MPM_F_EXX * (size_t)a->K);
+    AALLOC(a->seg_active, np * sizeof(int) * (size_t)a->K);
+    AALLOC(a->seg_st, sizeof(DevState) * (size_t)a->K);
+    AALLOC(a->lam_store, np * sizeof(double) * MPM_F_EXX);
+    for (int f = 0; f < MPM_F_EXX; ++f) a->A.lam[f] = a->lam_store + (size_t)f * np;
+    AALLOC(a->A.coef, np * sizeof(double) * kAdjCoef);
+    AALLOC(a->A.nadj, nn * sizeof(double) * 5);
+    AALLOC(a->A.partials, sizeof(double) * kAdjParams * (size_t)a->g2p_blocks);
+    AALLOC(a->acc, sizeof(double) * kAdjParams);
+#undef AALLOC
+    *out = a;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_adjoint_set_particles(mpm_hip_adjoint *a, const double *m, const double *v,
+                                  const double *x, const double *y, const double *vx,
+                                  const double *vy, const double *sxx, const double *sxy,
+                                  const double *syy)
+{
+    if (!a) return MPM_HIP_ERR_INVALID;
+    int rc = mpm_hip_set_particles(a->ctx, m, v, x, y, vx, vy, sxx, sxy, syy);
+    if (rc) return rc;
+    rc = adj_copy_state(a, true, a->init, a->init_active, a->init_st);
+    if (rc) return rc;
+    a->have_particles = true;
+    a->have_forward = false;
+    return mpm_hip_sync(a->ctx);
+}
+
+int mpm_hip_adjoint_set_material(mpm_hip_adjoint *a, const mpm_hip_material *mat)
+{
+    if (!a || !mat) return MPM_HIP_ERR_INVALID;
+    mpm_hip_config c = a->ctx->cfg;
+    c.material = *mat;
+    GridParams g;
+    Material mt;
+    if (params_from_config(&c, a->ctx->g.cap, g, mt) != MPM_HIP_OK) return MPM_HIP_ERR_INVALID;
+    a->ctx->cfg.material = *mat;
+    a->ctx->mt = mt;
+    a->have_forward = false;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_adjoint_forward(mpm_hip_adjoint *a, int nsteps)
+{
+    if (!a || nsteps < 0 || !a->have_particles) return MPM_HIP_ERR_INVALID;
+    mpm_hip_ctx *ctx = a->ctx;
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const int ncp = (nsteps + a->K - 1) / a->K;
+    if (ncp > a->ncp) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (a->cp) hipFree(a->cp);
+        if (a->cp_active) hipFree(a->cp_active);
+        if (a->cp_st) hipFree(a->cp_st);
+        a->cp = nullptr;
+        a->cp_active = nullptr;
+        a->cp_st = nullptr;
+        a->ncp = 0;
+        const size_t np = a->np ? a->np : 1;
+        if (hipMalloc((void **)&a->cp, np * sizeof(double) * MPM_F_EXX * (size_t)ncp) != hipSuccess
+            || hipMalloc((void **)&a->cp_active, np * sizeof(int) * (size_t)ncp) != hipSuccess
+            || hipMalloc((void **)&a->cp_st, sizeof(DevState) * (size_t)ncp) != hipSuccess) {
+            return MPM_HIP_ERR_ALLOC;
+        }
+        a->ncp = ncp;
+    }
+    rc = adj_copy_state(a, false, a->init, a->init_active, a->init_st);
+    if (rc) return rc;
+    const size_t np = a->np ? a->np : 1;
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % a->K == 0) {
+            const int c = s / a->K;
+            rc = adj_copy_state(a, true, a->cp + (size_t)c * MPM_F_EXX * np,
+                                a->cp_active + (size_t)c * np, a->cp_st + c);
+            if (rc) return rc;
+        }
+        if ((rc = launch_p2g(ctx)) || (rc = launch_g2p(ctx))) return rc;
+    }
+    a->nsteps = nsteps;
+    a->have_forward = true;
+    return mpm_hip_sync(ctx);
+}
+
+int mpm_hip_adjoint_get_field(mpm_hip_adjoint *a, int field, double *host)
+{
+    if (!a) return MPM_HIP_ERR_INVALID;
+    return mpm_hip_get_field(a->ctx, field, host);
+}
+
+int mpm_hip_adjoint_get_active(mpm_hip_adjoint *a, int *host)
+{
+    if (!a) return MPM_HIP_ERR_INVALID;
+    return mpm_hip_get_active(a->ctx, host);
+}
+
+int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
+                             double *grad_params, double *const *grad_init)
+{
+    if (!a || !a->have_forward) return MPM_HIP_ERR_INVALID;
+    mpm_hip_ctx *ctx = a->ctx;
+    int rc = use_device(ctx);
+    if (rc) return rc;
+    const size_t np = a->np ? a->np : 1;
+    const size_t bytes = a->np * sizeof(double);
+    for (int f = 0; f < MPM_F_EXX; ++f) {
+        if (seed && seed[f] && bytes) {
+            HIP_TRY(hipMemcpyAsync(a->A.lam[f], seed[f], bytes, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        } else {
+            HIP_TRY(hipMemsetAsync(a->A.lam[f], 0, np * sizeof(double), ctx->stream));
+        }
+    }
+    HIP_TRY(hipMemsetAsync(a->acc, 0, sizeof(double) * kAdjParams, ctx->stream));
+    const int nseg = (a->nsteps + a->K - 1) / a->K;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int c0 = sg * a->K;
+        const int len = (a->nsteps - c0) < a->K ? (a->nsteps - c0) : a->K;
+        rc = adj_copy_state(a, false, a->cp + (size_t)sg * MPM_F_EXX * np,
+                            a->cp_active + (size_t)sg * np, a->cp_st + sg);
+        if (rc) return rc;
+        for (int k = 0; k < len; ++k) {
+            rc = adj_copy_state(a, true, a->seg + (size_t)k * MPM_F_EXX * np,
+                                a->seg_active + (size_t)k * np, a->seg_st + k);
+            if (rc) return rc;
+            if (k + 1 < len) {
+                if ((rc = launch_p2g(ctx)) || (rc = launch_g2p(ctx))) return rc;
+            }
+        }
+        for (int k = len - 1; k >= 0; --k) {
+            rc = adj_copy_state(a, false, a->seg + (size_t)k * MPM_F_EXX * np,
+                                a->seg_active + (size_t)k * np, a->seg_st + k);
+            if (rc) return rc;
+            rc = adj_reverse_step(a);
+            if (rc) return rc;
+        }
+    }
+    double acc[kAdjParams];
+    HIP_TRY(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad_init && bytes) {
+        for (int f = 0; f < MPM_F_EXX; ++f) {
+            if (grad_init[f]) {
+                HIP_TRY(hipMemcpyAsync(grad_init[f], a->A.lam[f], bytes, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            }
+        }
+    }
+    rc = mpm_hip_sync(ctx);
+    if (rc) return rc;
+    if (grad_params) {
+        // G = E / (2 (1 + nu)), lambda = E / (3 (1 - 2 nu)) - 2 G / 3
+        const double E = ctx->cfg.material.E, nu = ctx->cfg.material.nu;
+        const double dG_dE = 1.0 / (2.0 * (1.0 + nu));
+        const double dK_dE = 1.0 / (3.0 * (1.0 - 2.0 * nu));
+        const double dG_dnu = -E / (2.0 * (1.0 + nu) * (1.0 + nu));
+        const double dK_dnu = 2.0 * E / (3.0 * (1.0 - 2.0 * nu) * (1.0 - 2.0 * nu));
+        grad_params[0] = acc[0] * dG_dE + acc[1] * (dK_dE - 2.0 / 3.0 * dG_dE);
+        grad_params[1] = acc[0] * dG_dnu + acc[1] * (dK_dnu - 2.0 / 3.0 * dG_dnu);
+        for (int k = 2; k < kAdjParams; ++k) grad_params[k] = acc[k];
+    }
+    a->have_forward = false;
+    return MPM_HIP_OK;
+}
+
+int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a)
+{
+    if (!a) return MPM_HIP_OK;
+    if (a->ctx) {
+        hipSetDevice(a->ctx->device);
+        hipStreamSynchronize(a->ctx->stream);
+    }
+    void *arrs[] = {a->init, a->init_active, a->init_st, a->cp, a->cp_active, a->cp_st,
+                    a->seg, a->seg_active, a->seg_st, a->lam_store, a->A.coef, a->A.nadj,
+                    a->A.partials, a->acc};
+    for (void *p : arrs) {
+        if (p) hipFree(p);
+    }
+    if (a->ctx) mpm_hip_destroy(a->ctx);
+    delete a;
     return MPM_HIP_OK;
 }
 

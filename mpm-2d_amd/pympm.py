@@ -107,6 +107,15 @@ def load():
     lib.mpm_hip_batch_time_steps.argtypes = [P, I, D]
     lib.mpm_hip_batch_autotune.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_batch_destroy.argtypes = [P]
+    DP = ctypes.POINTER(D)
+    lib.mpm_hip_adjoint_create.argtypes = [ctypes.POINTER(Config), Z, I, ctypes.POINTER(P)]
+    lib.mpm_hip_adjoint_set_particles.argtypes = [P] + [D] * 9
+    lib.mpm_hip_adjoint_set_material.argtypes = [P, ctypes.POINTER(Material)]
+    lib.mpm_hip_adjoint_forward.argtypes = [P, I]
+    lib.mpm_hip_adjoint_get_field.argtypes = [P, I, D]
+    lib.mpm_hip_adjoint_get_active.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    lib.mpm_hip_adjoint_backward.argtypes = [P, DP, D, DP]
+    lib.mpm_hip_adjoint_destroy.argtypes = [P]
     _lib = lib
     return lib
 
@@ -318,6 +327,92 @@ class Batch:
         if self.b:
             self.lib.mpm_hip_batch_destroy(self.b)
             self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+STATE_FIELDS = FIELDS[:14]
+PARAMS = ["E", "nu", "mu_s", "mu_2", "I_0", "grains_d", "grains_rho"]
+
+
+class Adjoint:
+    """Reverse-mode derivative of an n-step run (mpm_hip_adjoint_*)."""
+
+    def __init__(self, cfg, particles, checkpoint_every=20):
+        self.lib = load()
+        self.cfg = cfg
+        self.np = len(particles["x"])
+        self.a = ctypes.c_void_p()
+        _check(self.lib.mpm_hip_adjoint_create(ctypes.byref(cfg), self.np, checkpoint_every,
+                                               ctypes.byref(self.a)), "mpm_hip_adjoint_create")
+        self.set_particles(particles)
+
+    def set_particles(self, particles):
+        arrs = [np.ascontiguousarray(particles[k], dtype=np.float64)
+                for k in ("m", "v", "x", "y", "vx", "vy")]
+        stress = [particles.get(k) for k in ("sxx", "sxy", "syy")]
+        stress = [None if s is None else np.ascontiguousarray(s, dtype=np.float64) for s in stress]
+        ptrs = [_dptr(a) for a in arrs] + [None if s is None else _dptr(s) for s in stress]
+        _check(self.lib.mpm_hip_adjoint_set_particles(self.a, *ptrs),
+               "mpm_hip_adjoint_set_particles")
+
+    def set_material(self, **kw):
+        mat = Material()
+        for name, _ in Material._fields_:
+            setattr(mat, name, getattr(self.cfg.material, name))
+        for k, v in kw.items():
+            setattr(mat, k, float(v))
+        _check(self.lib.mpm_hip_adjoint_set_material(self.a, ctypes.byref(mat)),
+               "mpm_hip_adjoint_set_material")
+        self.cfg.material = mat
+
+    def forward(self, nsteps):
+        _check(self.lib.mpm_hip_adjoint_forward(self.a, nsteps), "mpm_hip_adjoint_forward")
+
+    def get(self, name):
+        out = np.empty(self.np, dtype=np.float64)
+        _check(self.lib.mpm_hip_adjoint_get_field(self.a, FIELD_ID[name], _dptr(out)),
+               f"mpm_hip_adjoint_get_field({name})")
+        return out
+
+    def active(self):
+        out = np.empty(self.np, dtype=np.int32)
+        _check(self.lib.mpm_hip_adjoint_get_active(
+            self.a, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))), "mpm_hip_adjoint_get_active")
+        return out
+
+    def backward(self, seed):
+        """seed: dict field -> dL/d(final field).  Returns (dL/d params as a
+        dict over PARAMS, dict of dL/d(initial field) over STATE_FIELDS)."""
+        n = len(STATE_FIELDS)
+        keep = []
+        seeds = (ctypes.POINTER(ctypes.c_double) * n)()
+        for i, k in enumerate(STATE_FIELDS):
+            if k in seed and seed[k] is not None:
+                a = np.ascontiguousarray(seed[k], dtype=np.float64)
+                keep.append(a)
+                seeds[i] = _dptr(a)
+        outs = {k: np.zeros(self.np) for k in STATE_FIELDS}
+        optr = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(outs[k]) for k in STATE_FIELDS])
+        g = np.zeros(len(PARAMS))
+        _check(self.lib.mpm_hip_adjoint_backward(self.a, seeds, _dptr(g), optr),
+               "mpm_hip_adjoint_backward")
+        return dict(zip(PARAMS, g)), outs
+
+    def close(self):
+        if self.a:
+            self.lib.mpm_hip_adjoint_destroy(self.a)
+            self.a = ctypes.c_void_p()
 
     def __del__(self):
         try:

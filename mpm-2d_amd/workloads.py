@@ -176,3 +176,23 @@ def ensemble(members=256, seed=11, p_min=2048, p_max=65536, sides=None):
                         material=mat, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
                         particles=block(n // 4, n // 2, n, seed=seed * 1000 + k)))
     return out
+
+
+def adjoint_block(seed=4, cells=128, material=None):
+    """Adjoint workload: 128 x 128 cells (129^2 nodes), a 64 x 64 cell block
+    (65,536 particles) at x in [0.25, 0.75), y in [0, 0.5) under full gravity
+    from the first step, with a lithostatic initial stress (k0 = 0.4, so
+    tau/p = 0.43 starts above mu_s) and an initial slide vx = 0.5 m/s over
+    the no-slip floor; box walls as in the column collapse, dt = 1e-5, 200
+    steps.  The target run uses mu_s = 0.42 and E = 2e6."""
+    n = cells
+    p = block(n // 2, n // 2, n, seed=seed, cell_x0=n // 4)
+    top = 0.5
+    p = lithostatic(p, lambda x: top + 0.0 * x, rho=1500.0, g=9.81, k0=0.4)
+    p["vx"] = np.full(len(p["x"]), 0.5)
+    return dict(name=f"adjoint_block_{n}", N=n + 1, h=1.0 / n, dt=1e-5, steps=200,
+                material=material or dict(BASELINE_MATERIAL), g_mag=9.81, ramp_time=1e-9,
+                **BOX_FRICTION, particles=p)
+
+
+ADJOINT_TARGET = dict(BASELINE_MATERIAL, mu_s=0.42, E=2e6)
