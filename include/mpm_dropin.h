@@ -33,6 +33,12 @@ job_t *mpm_init(int N, double h, particle_t *particles, size_t num_particles, do
  * EXIT_ERROR_THREADING, the reference's code for a failed worker set-up. */
 void explicit_mpm_step_usl_threaded(void *task);
 
+/* libmpm/process_usl.h:15, process_usl.c:284-462: the MUSL step, same
+ * calling convention (the reference driver's loop has it commented out,
+ * src/main.c:948).  A run uses one scheme: the step function called first
+ * fixes it. */
+void explicit_mpm_step_musl_threaded(void *task);
+
 /* libmpm/process.h:253, process_usl.c:974-1004 (also releases the device). */
 void mpm_cleanup(job_t *job);
 

@@ -104,9 +104,13 @@ typedef struct mpm_hip_nonlocal {
 
 enum mpm_hip_material_kind {
     MPM_MAT_G_LOCAL_MU2 = 0,   /* materialsrc/g_local_mu2.c */
-    MPM_MAT_G_NONLOCAL = 1     /* materialsrc/g_nonlocal.c: per-step FEM
+    MPM_MAT_G_NONLOCAL = 1,    /* materialsrc/g_nonlocal.c: per-step FEM
                                   fluidity solve, here Jacobi-preconditioned
                                   conjugate gradients on the device */
+    MPM_MAT_ISOLIN = 2         /* materialsrc/isolin.c: isotropic linear
+                                  elasticity with the Jaumann rate, on the
+                                  in-plane stress (material.E, material.nu;
+                                  MPM_F_TXX/TXY/TYY hold sxx/sxy/syy) */
 };
 
 typedef struct mpm_hip_config {
@@ -128,7 +132,18 @@ typedef struct mpm_hip_config {
     mpm_hip_nonlocal nonlocal;
     double cg_rel_tol;      /* g_nonlocal solve: stop at |r| <= tol |f| */
     int cg_max_iter;        /* g_nonlocal solve: iteration cap */
+    int scheme;             /* mpm_hip_scheme */
 } mpm_hip_config;
+
+enum mpm_hip_scheme {
+    MPM_SCHEME_USL = 0,     /* explicit_mpm_step_usl_threaded,
+                               libmpm/process_usl.c:465-592 */
+    MPM_SCHEME_MUSL = 1     /* explicit_mpm_step_musl_threaded,
+                               libmpm/process_usl.c:284-462: the particle
+                               momentum is mapped to the grid a second time
+                               and the strain rate taken from those node
+                               velocities (not with g_nonlocal) */
+};
 
 typedef struct mpm_hip_ctx mpm_hip_ctx;
 

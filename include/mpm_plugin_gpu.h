@@ -31,13 +31,15 @@ extern "C" {
 #define MPM_PLUGIN_GPU_ABI 1
 
 enum mpm_gpu_material_kind {
-    MPM_GPU_MATERIAL_G_LOCAL_MU2 = 1   /* materialsrc/g_local_mu2.c */
+    MPM_GPU_MATERIAL_G_LOCAL_MU2 = 1,  /* materialsrc/g_local_mu2.c */
+    MPM_GPU_MATERIAL_ISOLIN = 2        /* materialsrc/isolin.c */
 };
 
 typedef struct mpm_material_gpu_desc {
     int abi;        /* MPM_PLUGIN_GPU_ABI */
     int kind;       /* mpm_gpu_material_kind */
-    /* g_local_mu2: E, nu, mu_s, mu_2, I_0, grains_d, grains_rho, rho_crit */
+    /* g_local_mu2: E, nu, mu_s, mu_2, I_0, grains_d, grains_rho, rho_crit;
+     * isolin: E, nu */
     double params[8];
 } mpm_material_gpu_desc;
 

@@ -60,7 +60,8 @@ constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
 constexpr int kG2PThreads = 256;
 
 enum : int { kStatusBinOverflow = 1 };
-enum : int { kModeStoreAux = 1, kModeHostMaterial = 2, kModeNonlocal = 4 };
+enum : int { kModeStoreAux = 1, kModeHostMaterial = 2, kModeNonlocal = 4, kModeIsolin = 8,
+              kModeMuslMove = 16, kModeMuslFinish = 32 };
 
 struct DevState {
     double t;           // job->t
@@ -102,6 +103,7 @@ struct Particles {
     int *elem;          // element of the next step, -1 outside the grid
     int *active;
     int *elem_prev;     // g_nonlocal: element of the step just taken
+    double *xn, *yn;    // MUSL: positions after G2P, committed after the stress
 };
 
 struct alignas(64) NodeRec {
@@ -398,7 +400,10 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     bin_insert(e >= 0, e, p, cnt, bkt, g.cap, &st->status);
 }
 
-template <int CAP, bool BATCH>
+// MUSL2: the second mapping of the MUSL step (process_usl.c:409-440): only
+// the mass and momentum chains, then v = p / m without boundary conditions
+// or the force update.
+template <int CAP, bool BATCH, bool MUSL2 = false>
 __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
@@ -649,14 +654,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                     if (role == 0) {
                         am += sf * mp;
                         ap += sf * ps.x;
-                        af += sf * (bb * mp);
-                        af += b1 * ps.y;       // -sxx v
-                        af += b2 * msy.y;      // -sxy v
+                        if (!MUSL2) {
+                            af += sf * (bb * mp);
+                            af += b1 * ps.y;       // -sxx v
+                            af += b2 * msy.y;      // -sxy v
+                        }
                     } else {
                         ap += sf * ps.x;
-                        af += sf * (bb * mp);
-                        af += b1 * msy.y;      // -sxy v
-                        af += b2 * ps.y;       // -syy v
+                        if (!MUSL2) {
+                            af += sf * (bb * mp);
+                            af += b1 * msy.y;      // -sxy v
+                            af += b2 * ps.y;       // -syy v
+                        }
                     }
                 }
             }
@@ -678,6 +687,19 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     double amx = ap, afx = af;
     const double2 yv = s_y[nt];
     double amy = yv.x, afy = yv.y;
+    if (MUSL2) {
+        NodeRec r;
+        r.m = am;
+        r.mx = amx;
+        r.my = amy;
+        r.fx = 0.0;
+        r.fy = 0.0;
+        r.vx = am > kTol ? amx / am : 0.0;
+        r.vy = am > kTol ? amy / am : 0.0;
+        r.pad = 0.0;
+        nodes[(size_t)j * g.N + i] = r;
+        return;
+    }
 
     // bc_momentum then bc_force, then move_grid_split.
     const unsigned f = bc_fixed(g, i, j, st->t + g.dt, mask);
@@ -724,7 +746,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         bkt_next += (size_t)mi.cell_base * g.cap;
         st = bt.st + mb;
     }
-    if (p == first) {
+    // MUSL runs this kernel twice per step: the particle update (move), then,
+    // after the second mapping, strain rate, volume, stress and rebinning
+    // (finish); USL does both in one pass.
+    const bool move = !(mode & kModeMuslFinish);
+    const bool finish = !(mode & kModeMuslMove);
+    if (p == first && finish) {
         // t += dt (process_usl.c:522), then time_varying_loads for the next
         // step (src/main.c:975).  No thread of this kernel reads *st.
         const double t = st->t + g.dt;
@@ -758,28 +785,53 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
 
     // move_particles_explicit_usl_split
     double ax = 0.0, ay = 0.0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (nd[q].m < kTol) continue;
-        ax += ((sh.s[q] * nd[q].fx) / nd[q].m);
-        ay += ((sh.s[q] * nd[q].fy) / nd[q].m);
-    }
     double vx = P.vx[p], vy = P.vy[p];
-    vx += dt * ax;
-    vy += dt * ay;
-    double dux = 0.0, duy = 0.0;
+    double ux, uy;
+    if (move) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (nd[q].m < kTol) continue;
-        dux += ((sh.s[q] * nd[q].mx) / nd[q].m);
-        duy += ((sh.s[q] * nd[q].my) / nd[q].m);
+        for (int q = 0; q < 4; ++q) {
+            if (nd[q].m < kTol) continue;
+            ax += ((sh.s[q] * nd[q].fx) / nd[q].m);
+            ay += ((sh.s[q] * nd[q].fy) / nd[q].m);
+        }
+        vx += dt * ax;
+        vy += dt * ay;
+        double dux = 0.0, duy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (nd[q].m < kTol) continue;
+            dux += ((sh.s[q] * nd[q].mx) / nd[q].m);
+            duy += ((sh.s[q] * nd[q].my) / nd[q].m);
+        }
+        dux *= dt;
+        duy *= dt;
+        x += dux;
+        y += duy;
+        ux = P.ux[p] + dux;
+        uy = P.uy[p] + duy;
+        if (!finish) {
+            // MUSL: the second mapping needs the new velocities at the
+            // step's start positions; the new positions wait in xn, yn
+            P.vx[p] = vx;
+            P.vy[p] = vy;
+            P.ux[p] = ux;
+            P.uy[p] = uy;
+            P.xn[p] = x;
+            P.yn[p] = y;
+            if (mode & kModeStoreAux) {
+                P.ax[p] = ax;
+                P.ay[p] = ay;
+                P.xl[p] = sh.xl;
+                P.yl[p] = sh.yl;
+            }
+            return;
+        }
+    } else {
+        x = P.xn[p];
+        y = P.yn[p];
+        ux = P.ux[p];
+        uy = P.uy[p];
     }
-    dux *= dt;
-    duy *= dt;
-    x += dux;
-    y += duy;
-    const double ux = P.ux[p] + dux;
-    const double uy = P.uy[p] + duy;
 
     // calculate_strainrate_split (SMEAR4 folds left to right)
     const double exx = 1.0 * (sh.b1[0] * nd[0].vx + sh.b1[1] * nd[1].vx
@@ -804,6 +856,20 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         P.gf_bulk[p] = gfb;
         P.xisq_inv[p] = xsi;
         P.elem_prev[p] = e;
+    } else if (mode & kModeIsolin) {
+        // isolin calculate_stress_threaded (materialsrc/isolin.c:89-110) on
+        // sxx, sxy, syy (held in the Txx, Txy, Tyy arrays)
+        const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
+        const double trD = exx + eyy;
+        double dsjxx = mt.lambda * trD + mt.two_G * exx;
+        double dsjxy = mt.two_G * exy;
+        double dsjyy = mt.lambda * trD + mt.two_G * eyy;
+        dsjxx += 2 * wxy * sxy;
+        dsjxy -= wxy * (sxx - syy);
+        dsjyy -= 2 * wxy * sxy;
+        P.txx[p] = sxx + dt * dsjxx;
+        P.txy[p] = sxy + dt * dsjxy;
+        P.tyy[p] = syy + dt * dsjyy;
     } else if (!(mode & kModeHostMaterial)) {
         double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
         double gp = P.gp[p], gdp;
@@ -827,10 +893,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         P.exy[p] = exy;
         P.eyy[p] = eyy;
         P.wxy[p] = wxy;
-        P.ax[p] = ax;
-        P.ay[p] = ay;
-        P.xl[p] = sh.xl;
-        P.yl[p] = sh.yl;
+        if (move) {
+            P.ax[p] = ax;
+            P.ay[p] = ay;
+            P.xl[p] = sh.xl;
+            P.yl[p] = sh.yl;
+        }
     }
 
     // The next step's create_particle_to_element_map, and its bin.
@@ -1821,27 +1889,33 @@ static int use_device(const mpm_hip_ctx *ctx)
     return MPM_HIP_OK;
 }
 
-static int launch_p2g(mpm_hip_ctx *ctx)
+template <bool MUSL2>
+static void p2g_kernel_launch(mpm_hip_ctx *ctx)
 {
     const int cur = ctx->cur, nxt = 1 - cur;
     const dim3 grid(ctx->p2g_blocks), block(kP2GThreads);
     switch (ctx->g.cap) {
     case 16:
-        k_p2g_grid<16, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<16, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, BatchTables{});
         break;
     case 32:
-        k_p2g_grid<32, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<32, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, BatchTables{});
         break;
     default:
-        k_p2g_grid<64, false><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<64, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, BatchTables{});
         break;
     }
+}
+
+static int launch_p2g(mpm_hip_ctx *ctx)
+{
+    p2g_kernel_launch<false>(ctx);
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
 }
@@ -1885,6 +1959,18 @@ static int launch_g2p(mpm_hip_ctx *ctx)
     if (ctx->cfg.flags & (MPM_HIP_STORE_AUX | MPM_HIP_HOST_MATERIAL)) mode |= kModeStoreAux;
     if (ctx->cfg.flags & MPM_HIP_HOST_MATERIAL) mode |= kModeHostMaterial;
     if (ctx->nonlocal) mode |= kModeNonlocal | kModeStoreAux;
+    if (ctx->cfg.material_kind == MPM_MAT_ISOLIN && !(mode & kModeHostMaterial)) {
+        mode |= kModeIsolin;
+    }
+    if (ctx->cfg.scheme == MPM_SCHEME_MUSL) {
+        // particle update; second mapping of the momentum; strain rate,
+        // volume, stress and rebinning from the re-mapped velocities
+        k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+            ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+            (int)ctx->np, mode | kModeMuslMove, BatchTables{});
+        p2g_kernel_launch<true>(ctx);
+        mode |= kModeMuslFinish;
+    }
     k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
         (int)ctx->np, mode, BatchTables{});
@@ -1982,7 +2068,7 @@ static int batch_eager_step(mpm_hip_batch *b)
 static void batch_drop_graph(mpm_hip_batch *b)
 {
     if (b->exec) {
-        hipGraphExecDestroy(b->exec);
+        (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
 }
@@ -2002,11 +2088,11 @@ static int batch_capture(mpm_hip_batch *b)
     hipGraph_t graph = nullptr;
     const hipError_t end = hipStreamEndCapture(b->stream, &graph);
     if (rc || end != hipSuccess || !graph) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc ? rc : MPM_HIP_ERR_DEVICE;
     }
     const hipError_t inst = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (inst != hipSuccess) {
         b->exec = nullptr;
         return MPM_HIP_ERR_DEVICE;
@@ -2109,7 +2195,8 @@ static int params_from_config(const mpm_hip_config *cfg, int cap, GridParams &g,
     mt.nl_rho = cfg->nonlocal.rho;
     mt.nl_p_cap = cfg->nonlocal.p_cap;
     mt.nl_c = cfg->nonlocal.c;
-    if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL) {
+    if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL
+        && cfg->material_kind != MPM_MAT_ISOLIN) {
         return MPM_HIP_ERR_INVALID;
     }
     return MPM_HIP_OK;
@@ -2148,6 +2235,7 @@ void mpm_hip_default_config(mpm_hip_config *cfg)
     cfg->nonlocal.c = 1e-2;
     cfg->cg_rel_tol = 1e-10;
     cfg->cg_max_iter = 20000;
+    cfg->scheme = MPM_SCHEME_USL;
     cfg->bin_capacity = 0;
     cfg->flags = 0;
     cfg->device = 0;
@@ -2201,6 +2289,11 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
         return MPM_HIP_ERR_INVALID;
     }
     ctx->nonlocal = cfg->material_kind == MPM_MAT_G_NONLOCAL;
+    if ((cfg->scheme != MPM_SCHEME_USL && cfg->scheme != MPM_SCHEME_MUSL)
+        || (cfg->scheme == MPM_SCHEME_MUSL && ctx->nonlocal)) {
+        delete ctx;
+        return MPM_HIP_ERR_INVALID;
+    }
     if (ctx->nonlocal && ((cfg->flags & MPM_HIP_HOST_MATERIAL) || !(cfg->cg_rel_tol >= 0.0)
                           || cfg->cg_max_iter < 0)) {
         delete ctx;
@@ -2237,6 +2330,10 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
         return MPM_HIP_ERR_ALLOC;
     }
     ALLOC_OR_FAIL(P.elem, np * sizeof(int));
+    if (cfg->scheme == MPM_SCHEME_MUSL) {
+        ALLOC_OR_FAIL(P.xn, np * sizeof(double));
+        ALLOC_OR_FAIL(P.yn, np * sizeof(double));
+    }
     if (ctx->nonlocal) {
         ALLOC_OR_FAIL(P.elem_prev, np * sizeof(int));
         NlSystem &S = ctx->nl;
@@ -2568,18 +2665,18 @@ int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms)
         double p2g = 0, g2p = 0;
         float a = 0.f;
         for (int s = 0; s < n; ++s) {
-            hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]);
+            (void)hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]);
             p2g += a;
-            hipEventElapsedTime(&a, ev[2 * s + 1], ev[2 * s + 2]);
+            (void)hipEventElapsedTime(&a, ev[2 * s + 1], ev[2 * s + 2]);
             g2p += a;
         }
-        hipEventElapsedTime(&a, ev[0], ev[2 * n]);
+        (void)hipEventElapsedTime(&a, ev[0], ev[2 * n]);
         ms[0] = a;
         ms[1] = p2g;
         ms[2] = g2p;
     }
     for (int k = 0; k < created; ++k) {
-        hipEventDestroy(ev[k]);
+        (void)hipEventDestroy(ev[k]);
     }
     free(ev);
     if (rc) return rc;
@@ -2591,33 +2688,35 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx)
     if (!ctx) {
         return MPM_HIP_OK;
     }
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     if (ctx->stream) {
-        hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
     }
     for (int f = 0; f < MPM_F_COUNT; ++f) {
-        if (ctx->pfield[f]) hipFree(ctx->pfield[f]);
+        if (ctx->pfield[f]) (void)hipFree(ctx->pfield[f]);
     }
-    if (ctx->P.elem) hipFree(ctx->P.elem);
-    if (ctx->P.elem_prev) hipFree(ctx->P.elem_prev);
+    if (ctx->P.elem) (void)hipFree(ctx->P.elem);
+    if (ctx->P.xn) (void)hipFree(ctx->P.xn);
+    if (ctx->P.yn) (void)hipFree(ctx->P.yn);
+    if (ctx->P.elem_prev) (void)hipFree(ctx->P.elem_prev);
     {
         NlSystem &S = ctx->nl;
         double *arrs[] = {S.K, S.f, S.x, S.r, S.p, S.u, S.w, S.s, S.dinv, S.partials, S.res};
         for (double *a : arrs) {
-            if (a) hipFree(a);
+            if (a) (void)hipFree(a);
         }
-        if (S.stats) hipFree(S.stats);
-        if (S.barrier) hipFree(S.barrier);
+        if (S.stats) (void)hipFree(S.stats);
+        if (S.barrier) (void)hipFree(S.barrier);
     }
-    if (ctx->P.active) hipFree(ctx->P.active);
-    if (ctx->nodes) hipFree(ctx->nodes);
-    if (ctx->mask) hipFree(ctx->mask);
+    if (ctx->P.active) (void)hipFree(ctx->P.active);
+    if (ctx->nodes) (void)hipFree(ctx->nodes);
+    if (ctx->mask) (void)hipFree(ctx->mask);
     for (int b = 0; b < 2; ++b) {
-        if (ctx->cnt[b]) hipFree(ctx->cnt[b]);
-        if (ctx->bkt[b]) hipFree(ctx->bkt[b]);
+        if (ctx->cnt[b]) (void)hipFree(ctx->cnt[b]);
+        if (ctx->bkt[b]) (void)hipFree(ctx->bkt[b]);
     }
-    if (ctx->st) hipFree(ctx->st);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->st) (void)hipFree(ctx->st);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MPM_HIP_OK;
 }
@@ -2663,6 +2762,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         if (c->N < 2 || !(c->h > 0.0) || !(c->dt > 0.0) || c->N > 46340
             || num_particles[k] > (size_t)0x7fffffff || c->bc_kind == MPM_BC_MASK
             || c->material_kind != MPM_MAT_G_LOCAL_MU2 || (c->flags & MPM_HIP_HOST_MATERIAL)
+            || c->scheme != MPM_SCHEME_USL
             || (c->bin_capacity && c->bin_capacity != cap)) {
             mpm_hip_batch_destroy(b);
             return MPM_HIP_ERR_INVALID;
@@ -2966,15 +3066,15 @@ int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms)
     if (!rc && hipEventSynchronize(ev[3]) != hipSuccess) rc = MPM_HIP_ERR_DEVICE;
     if (!rc) {
         float a = 0.f;
-        hipEventElapsedTime(&a, ev[0], ev[3]);
+        (void)hipEventElapsedTime(&a, ev[0], ev[3]);
         ms[0] = a;
-        hipEventElapsedTime(&a, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&a, ev[1], ev[2]);
         ms[1] = a;
-        hipEventElapsedTime(&a, ev[2], ev[3]);
+        (void)hipEventElapsedTime(&a, ev[2], ev[3]);
         ms[2] = a;
     }
     for (int k = 0; k < 4; ++k) {
-        if (ev[k]) hipEventDestroy(ev[k]);
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
     }
     if (rc) return rc;
     return mpm_hip_batch_sync(b);
@@ -2986,7 +3086,7 @@ static int batch_set_capacity(mpm_hip_batch *b, int cap)
     batch_drop_graph(b);
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (int k = 0; k < 2; ++k) {
-        if (b->bkt[k]) hipFree(b->bkt[k]);
+        if (b->bkt[k]) (void)hipFree(b->bkt[k]);
         b->bkt[k] = nullptr;
         HIP_TRY(hipMalloc((void **)&b->bkt[k], b->ncells_total * (size_t)cap * sizeof(int)));
     }
@@ -3011,8 +3111,8 @@ int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, doubl
     HIP_TRY(hipMalloc((void **)&save, np * sizeof(double) * MPM_F_EXX));
     if (hipMalloc((void **)&save_i, np * sizeof(int) * 2) != hipSuccess
         || hipMalloc((void **)&save_st, sizeof(DevState) * (size_t)b->nmem) != hipSuccess) {
-        hipFree(save);
-        if (save_i) hipFree(save_i);
+        (void)hipFree(save);
+        if (save_i) (void)hipFree(save_i);
         return MPM_HIP_ERR_ALLOC;
     }
     auto copy_state = [&](bool to_save) -> hipError_t {
@@ -3061,9 +3161,9 @@ int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, doubl
         rc = batch_set_capacity(b, best >= 0 ? caps[best] : 64);
     }
     if (!rc) rc = mpm_hip_batch_sync(b);
-    hipFree(save);
-    hipFree(save_i);
-    hipFree(save_st);
+    (void)hipFree(save);
+    (void)hipFree(save_i);
+    (void)hipFree(save_st);
     if (rc) return rc;
     if (best < 0) return MPM_HIP_ERR_BIN_OVERFLOW;
     if (chosen) *chosen = caps[best];
@@ -3076,18 +3176,18 @@ int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, doubl
 int mpm_hip_batch_destroy(mpm_hip_batch *b)
 {
     if (!b) return MPM_HIP_OK;
-    hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
     for (int f = 0; f < MPM_F_EXX; ++f) {
-        if (b->pfield[f]) hipFree(b->pfield[f]);
+        if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
     void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->cnt[0], b->cnt[1], b->bkt[0],
                     b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member};
     for (void *a : arrs) {
-        if (a) hipFree(a);
+        if (a) (void)hipFree(a);
     }
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete[] b->hmem;
     delete[] b->cfgs;
     delete[] b->have;
@@ -3174,6 +3274,7 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out)
 {
     if (!cfg || !out || checkpoint_every <= 0 || cfg->material_kind != MPM_MAT_G_LOCAL_MU2
+        || cfg->scheme != MPM_SCHEME_USL
         || (cfg->flags & MPM_HIP_HOST_MATERIAL)) {
         return MPM_HIP_ERR_INVALID;
     }
@@ -3252,9 +3353,9 @@ int mpm_hip_adjoint_forward(mpm_hip_adjoint *a, int nsteps)
     const int ncp = (nsteps + a->K - 1) / a->K;
     if (ncp > a->ncp) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (a->cp) hipFree(a->cp);
-        if (a->cp_active) hipFree(a->cp_active);
-        if (a->cp_st) hipFree(a->cp_st);
+        if (a->cp) (void)hipFree(a->cp);
+        if (a->cp_active) (void)hipFree(a->cp_active);
+        if (a->cp_st) (void)hipFree(a->cp_st);
         a->cp = nullptr;
         a->cp_active = nullptr;
         a->cp_st = nullptr;
@@ -3368,14 +3469,14 @@ int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a)
 {
     if (!a) return MPM_HIP_OK;
     if (a->ctx) {
-        hipSetDevice(a->ctx->device);
-        hipStreamSynchronize(a->ctx->stream);
+        (void)hipSetDevice(a->ctx->device);
+        (void)hipStreamSynchronize(a->ctx->stream);
     }
     void *arrs[] = {a->init, a->init_active, a->init_st, a->cp, a->cp_active, a->cp_st,
                     a->seg, a->seg_active, a->seg_st, a->lam_store, a->A.coef, a->A.nadj,
                     a->A.partials, a->acc};
     for (void *p : arrs) {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
     if (a->ctx) mpm_hip_destroy(a->ctx);
     delete a;
@@ -3416,9 +3517,9 @@ int mpm_hip_eval_ops(int op, const double *a, const double *b, double *out, size
             rc = MPM_HIP_ERR_DEVICE;
         }
     }
-    if (da) hipFree(da);
-    if (db) hipFree(db);
-    if (dout) hipFree(dout);
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
     return rc;
 }
 

@@ -48,6 +48,8 @@ typedef struct gpu_job {
     mpm_hip_ctx *ctx;
     int ready;
     int host_material;
+    int isolin;             /* device isolin: the stress lives in sxx, sxy, syy */
+    int scheme;             /* MPM_SCHEME_USL or MPM_SCHEME_MUSL, fixed by the first step */
     int host_bc;
     int aux_on;
     uint8_t *mask;        /* effective Dirichlet mask last uploaded */
@@ -490,6 +492,7 @@ static void gpu_setup(job_t *job)
     cfg.loads_t0 = job->step_start_time;
     cfg.g_mag = G_MAG;
     cfg.ramp_time = RAMP_TIME;
+    cfg.scheme = G.scheme;
     const char *dev = getenv("MPM_HIP_DEVICE");
     cfg.device = dev ? atoi(dev) : 0;
     const char *cap = getenv("MPM_HIP_BIN_CAPACITY");
@@ -499,8 +502,14 @@ static void gpu_setup(job_t *job)
         symbol_beside((void *)job->material.calculate_stress_threaded, "material_gpu_descriptor");
     mpm_material_gpu_desc md;
     memset(&md, 0, sizeof md);
-    if (mdesc && mdesc(job, &md) && md.abi == MPM_PLUGIN_GPU_ABI
-        && md.kind == MPM_GPU_MATERIAL_G_LOCAL_MU2) {
+    int have = mdesc && mdesc(job, &md) && md.abi == MPM_PLUGIN_GPU_ABI;
+    if (have && md.kind == MPM_GPU_MATERIAL_ISOLIN) {
+        cfg.material_kind = MPM_MAT_ISOLIN;
+        cfg.material.E = md.params[0];
+        cfg.material.nu = md.params[1];
+        G.host_material = 0;
+        G.isolin = 1;
+    } else if (have && md.kind == MPM_GPU_MATERIAL_G_LOCAL_MU2) {
         cfg.material.E = md.params[0];
         cfg.material.nu = md.params[1];
         cfg.material.mu_s = md.params[2];
@@ -552,9 +561,12 @@ static void gpu_setup(job_t *job)
         y[i] = p->y;
         vx[i] = p->x_t;
         vy[i] = p->y_t;
-        sxx[i] = G.host_material ? p->sxx : p->T[XX];
-        sxy[i] = G.host_material ? p->sxy : p->T[XY];
-        syy[i] = G.host_material ? p->syy : p->T[YY];
+        /* the device law's stress: isolin and host laws act on s, g_local_mu2
+         * on T (which its material_init set from s) */
+        const int on_s = G.host_material || G.isolin;
+        sxx[i] = on_s ? p->sxx : p->T[XX];
+        sxy[i] = on_s ? p->sxy : p->T[XY];
+        syy[i] = on_s ? p->syy : p->T[YY];
     }
     CHECK(mpm_hip_set_particles(G.ctx, m, v, x, y, vx, vy, sxx, sxy, syy));
     const struct { int f; size_t off; } extra[] = {
@@ -633,6 +645,13 @@ static void download_kinematics(void)
 static void download_stress(void)
 {
     job_t *job = G.job;
+    if (G.isolin) {
+        /* isolin keeps its stress in s only (materialsrc/isolin.c:107-109) */
+        get_into(MPM_F_TXX, POFF(sxx));
+        get_into(MPM_F_TXY, POFF(sxy));
+        get_into(MPM_F_TYY, POFF(syy));
+        return;
+    }
     get_into(MPM_F_TXX, TOFF(XX));
     get_into(MPM_F_TXY, TOFF(XY));
     get_into(MPM_F_TYY, TOFF(YY));
@@ -696,11 +715,14 @@ static void serial_step(job_t *job)
     }
 }
 
-void explicit_mpm_step_usl_threaded(void *_task)
+static void step_threaded(void *_task, int scheme)
 {
     threadtask_t *task = (threadtask_t *)_task;
     job_t *job = task->job;
     int rc = pthread_barrier_wait(job->serialize_barrier);
+    if (rc == PTHREAD_BARRIER_SERIAL_THREAD && !G.ready) {
+        G.scheme = scheme;   /* the first step fixes the scheme of the run */
+    }
     if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
         serial_step(job);
     }
@@ -717,4 +739,14 @@ void explicit_mpm_step_usl_threaded(void *_task)
         }
         pthread_barrier_wait(job->serialize_barrier);
     }
+}
+
+void explicit_mpm_step_usl_threaded(void *task)
+{
+    step_threaded(task, MPM_SCHEME_USL);
+}
+
+void explicit_mpm_step_musl_threaded(void *task)
+{
+    step_threaded(task, MPM_SCHEME_MUSL);
 }

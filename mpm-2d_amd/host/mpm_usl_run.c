@@ -1,23 +1,36 @@
 /*
- * mpm_usl_run -- runs the USL loop the way the reference driver does
- * (src/main.c:164-982), against whatever libmpm it is linked with, so the
- * drop-in library can be exercised end to end without libconfuse (absent
- * here; the reference driver needs it for its .cfg parsing).
+ * mpm_usl_run -- the reference driver's set-up and loop (src/main.c:164-982)
+ * over whatever libmpm it is linked with, so the drop-in library runs end
+ * to end without libconfuse (absent here; the reference driver needs it for
+ * its .cfg parsing).  Two ways to call it:
+ *
+ *   mpm_usl_run -c SIM.cfg [-o DIR] [-p PFILE] [-g GFILE] [-u MAT.so]
+ *               [-b BC.so] [-t THREADS] [--musl] [--state FILE] [--check-cfg]
+ *               [t_max]
+ *     the reference's command line (src/main.c:64, 145-160; t_max defaults
+ *     to 1) and configuration file (host/cfg.h reads the libconfuse subset
+ *     the reference's .cfg files use; options and defaults as
+ *     src/main.c:169-251).  Writes info.txt, the frame file, the element,
+ *     state and log files into the output directory as src/main.c:660-715
+ *     opens them; frames in the reference's format (src/writer.c:169-201)
+ *     and the final state in write_state's format (writer.c:266-380).
  *
  *   mpm_usl_run -g GRID -p PARTICLES -u MATERIAL.so -b BC.so [-t THREADS]
  *               --dt DT --steps K [--mprops a,b,..] [--bprops a,b,..]
  *               [--biprops i,j,..] [--sample-rate HZ] [--frames FILE]
- *               [--state FILE]
+ *               [--musl] [--state FILE]
+ *     the same loop with t_stop = (K - 1/2) dt, so it runs exactly K steps.
  *
+ * --check-cfg prints the settings a -c run would use and exits.
+ * --musl steps with explicit_mpm_step_musl_threaded (process_usl.c:284-462;
+ * the reference's loop has that call commented out, src/main.c:948).
+ * --state writes the final particle state with full-precision hex floats,
+ * one particle per line:
+ *   x y x_t y_t v sxx sxy syy T[ZZ] state[9] state[10] ux uy active
  * Inputs use the reference's grid and particle file formats
  * (src/reader.c:17-119).  Plugins are loaded with dlopen/dlsym by the same
  * symbol names (src/main.c:427-456, 500-543).  The job is split over T
- * threads as in src/main.c:747-804; the loop is mpm_run_until
- * (src/main.c:934-982) with t_stop = (K - 1/2) dt so it runs exactly K
- * steps.  --frames writes frames in the reference's text format
- * (src/writer.c:169-201); --state writes the final particle state with
- * full-precision hex floats, one particle per line:
- *   x y x_t y_t v sxx sxy syy T[ZZ] state[9] state[10] ux uy active
+ * threads as in src/main.c:747-804.
  */
 #define _GNU_SOURCE
 #include <dlfcn.h>
@@ -27,12 +40,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <sys/stat.h>
+
+#include "cfg.h"
 #include "mpm_abi.h"
 #include "mpm_dropin.h"
 
 #define MINZ(a, b) ((a) < (b) ? (a) : (b))
 
 static FILE *frames_fd;
+static int use_musl;
 
 static int read_grid(const char *path, int *N, double *len)
 {
@@ -115,7 +132,11 @@ static void *run_until(void *arg)
     threadtask_t *task = arg;
     job_t *job = task->job;
     while (job->t < job->t_stop) {
-        explicit_mpm_step_usl_threaded(task);
+        if (use_musl) {
+            explicit_mpm_step_musl_threaded(task);
+        } else {
+            explicit_mpm_step_usl_threaded(task);
+        }
         int rc = pthread_barrier_wait(job->serialize_barrier);
         if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
             job->stepcount++;
@@ -129,57 +150,80 @@ static void *run_until(void *arg)
     return NULL;
 }
 
-int main(int argc, char **argv)
+/* write_state (src/writer.c:266-380): the format the reference's restart
+ * reader expects. */
+static void write_state(FILE *fd, const job_t *job)
 {
-    const char *grid = NULL, *pfile = NULL, *mat = NULL, *bc = NULL;
-    const char *frames = NULL, *state = NULL;
-    double dt = 0, rate = 60.0;
-    int steps = 0;
-    size_t T = 1;
-    double mprops[16], bprops[16], biprops_d[16];
-    size_t nm = 0, nb = 0, nbi = 0;
-    for (int a = 1; a < argc; a++) {
-        const char *k = argv[a];
-        const char *v = a + 1 < argc ? argv[a + 1] : "";
-        if (!strcmp(k, "-g")) grid = v, a++;
-        else if (!strcmp(k, "-p")) pfile = v, a++;
-        else if (!strcmp(k, "-u")) mat = v, a++;
-        else if (!strcmp(k, "-b")) bc = v, a++;
-        else if (!strcmp(k, "-t")) T = (size_t)atoi(v), a++;
-        else if (!strcmp(k, "--dt")) dt = strtod(v, NULL), a++;
-        else if (!strcmp(k, "--steps")) steps = atoi(v), a++;
-        else if (!strcmp(k, "--mprops")) nm = parse_list(v, mprops, 16), a++;
-        else if (!strcmp(k, "--bprops")) nb = parse_list(v, bprops, 16), a++;
-        else if (!strcmp(k, "--biprops")) nbi = parse_list(v, biprops_d, 16), a++;
-        else if (!strcmp(k, "--sample-rate")) rate = strtod(v, NULL), a++;
-        else if (!strcmp(k, "--frames")) frames = v, a++;
-        else if (!strcmp(k, "--state")) state = v, a++;
-        else {
-            fprintf(stderr, "unknown option %s\n", k);
-            return EXIT_ERROR_CFG_PARSE;
+    fprintf(fd, "%lg %lg %lg\n", job->t, job->dt, job->t_stop);
+    fprintf(fd, "%zu %zu %zu\n", job->num_particles, job->num_nodes, job->num_elements);
+    fprintf(fd, "%zu %lg\n", job->N, job->h);
+    for (size_t i = 0; i < job->num_particles; i++) {
+        const particle_t *p = &job->particles[i];
+        fprintf(fd, "%lg %lg\n", p->x, p->y);
+        fprintf(fd, "%lg %lg\n", p->xl, p->yl);
+        fprintf(fd, "%lg %lg\n", p->x_t, p->y_t);
+        fprintf(fd, "%lg\n", p->m);
+        fprintf(fd, "%lg\n", p->v);
+        fprintf(fd, "%lg\n", p->v0);
+        fprintf(fd, "%lg %lg %lg\n", p->sxx, p->sxy, p->syy);
+        fprintf(fd, "%lg %lg %lg %lg\n", p->exx_t, p->exy_t, p->eyy_t, p->wxy_t);
+        fprintf(fd, "%lg %lg\n", p->bx, p->by);
+        fprintf(fd, "%lg %lg %lg %lg\n", p->Fxx, p->Fxy, p->Fyx, p->Fyy);
+        fprintf(fd, "%lg %lg\n", p->ux, p->uy);
+        fprintf(fd, "%lg\n", p->color);
+        fprintf(fd, "%d\n", DEPVAR);
+        for (int j = 0; j < DEPVAR; j++) {
+            fprintf(fd, "%lg\n", p->state[j]);
         }
+        fprintf(fd, "%d\n", job->active[i]);
     }
-    if (!grid || !pfile || !mat || !bc || dt <= 0 || steps <= 0 || T < 1) {
-        fprintf(stderr, "usage: mpm_usl_run -g GRID -p PARTICLES -u MAT.so -b BC.so "
-                "--dt DT --steps K [-t T] [--mprops ..] [--bprops ..] [--biprops ..]\n");
-        return EXIT_ERROR_CFG_PARSE;
+    for (size_t i = 0; i < job->num_nodes; i++) {
+        fprintf(fd, "%lg %lg\n", job->nodes[i].x, job->nodes[i].y);
     }
-    int N;
-    double len;
-    particle_t *pdata;
-    size_t np;
-    if (read_grid(grid, &N, &len) || read_particles(pfile, &pdata, &np)) {
-        fprintf(stderr, "error reading inputs\n");
-        return EXIT_ERROR_CFG_PARSE;
+    for (size_t i = 0; i < job->num_elements; i++) {
+        fprintf(fd, "%d %d %d %d\n", job->elements[i].nodes[0], job->elements[i].nodes[1],
+                job->elements[i].nodes[2], job->elements[i].nodes[3]);
     }
-    double h = len / (N - 1);
-    job_t *job = mpm_init(N, h, pdata, np, (steps - 0.5) * dt);
+}
 
+static void write_hex_state(const char *path, const job_t *job)
+{
+    FILE *fd = fopen(path, "w");
+    if (!fd) {
+        fprintf(stderr, "cannot write '%s'\n", path);
+        return;
+    }
+    fprintf(fd, "%a %d\n", job->t, job->stepcount);
+    for (size_t i = 0; i < job->num_particles; i++) {
+        const particle_t *p = &job->particles[i];
+        fprintf(fd, "%a %a %a %a %a %a %a %a %a %a %a %a %a %d\n", p->x, p->y, p->x_t, p->y_t,
+                p->v, p->sxx, p->sxy, p->syy, p->T[ZZ], p->state[9], p->state[10], p->ux,
+                p->uy, job->active[i]);
+    }
+    fclose(fd);
+}
+
+typedef struct run_opts {
+    const char *cfgfile, *outdir, *grid, *pfile, *mat, *bc, *frames, *state;
+    double dt, rate, t_stop;
+    int steps, have_t_stop, check_cfg;
+    size_t T;
+    double mprops[64], bprops[64], biprops_d[64], miprops_d[64];
+    size_t nm, nb, nbi, nmi;
+} run_opts;
+
+static int load_plugins(job_t *job, const char *mat, const char *bc)
+{
     void *mh = dlopen(mat, RTLD_LAZY);
-    void *bh = dlopen(bc, RTLD_LAZY);
-    if (!mh || !bh) {
-        fprintf(stderr, "Can't dlopen() plugin: %s\n", dlerror());
+    if (!mh) {
+        fprintf(stderr, "Can't dlopen() material file '%s': %s.\n", mat, dlerror());
         return EXIT_ERROR_MATERIAL_FILE;
+    }
+    void *bh = dlopen(bc, RTLD_LAZY);
+    if (!bh) {
+        fprintf(stderr, "FATAL -- Can't dlopen() boundary condition file '%s': %s.\n", bc,
+                dlerror());
+        return EXIT_ERROR_BC_PROPS;
     }
     *(void **)&job->material.material_init = dlsym(mh, "material_init");
     *(void **)&job->material.calculate_stress = dlsym(mh, "calculate_stress");
@@ -189,23 +233,219 @@ int main(int argc, char **argv)
     *(void **)&job->boundary.bc_time_varying = dlsym(bh, "bc_time_varying");
     *(void **)&job->boundary.bc_momentum = dlsym(bh, "bc_momentum");
     *(void **)&job->boundary.bc_force = dlsym(bh, "bc_force");
-    if (!job->material.material_init || !job->material.calculate_stress_threaded
-        || !job->boundary.bc_validate || !job->boundary.bc_time_varying
-        || !job->boundary.bc_momentum || !job->boundary.bc_force) {
-        fprintf(stderr, "missing plugin symbol\n");
+    if (!job->material.material_init || !job->material.calculate_stress
+        || !job->material.calculate_stress_threaded) {
+        fprintf(stderr, "Error loading the material's symbols from '%s'.\n", mat);
         return EXIT_ERROR_MATERIAL_FILE;
     }
-    job->material.num_fp64_props = nm;
-    job->material.fp64_props = mprops;
-    job->boundary.num_fp64_props = nb;
-    job->boundary.fp64_props = bprops;
-    int biprops[16];
-    for (size_t i = 0; i < nbi; i++) biprops[i] = (int)biprops_d[i];
-    job->boundary.num_int_props = nbi;
-    job->boundary.int_props = biprops;
-    job->output.sample_rate_hz = rate;
-    job->dt = dt;
+    if (!job->boundary.bc_init || !job->boundary.bc_validate || !job->boundary.bc_time_varying
+        || !job->boundary.bc_momentum || !job->boundary.bc_force) {
+        fprintf(stderr, "FATAL -- No boundary conditions.\n");
+        return EXIT_ERROR_BC_PROPS;
+    }
+    return 0;
+}
 
+/* The reference's .cfg options (src/main.c:169-251, 380-730) into `o` and
+ * the job's output section; returns 0 or an exit code. */
+static int apply_cfg(const cfg_file *cf, run_opts *o, job_t *job)
+{
+    (void)job;
+    if (!o->mat) {
+        if (cfg_int(cf, "material|use-builtin", 1) != 0) {
+            fprintf(stderr, "The build has no builtin material: set material.use-builtin = 0 "
+                    "and material.material-file.\n");
+            return EXIT_ERROR_MATERIAL_FILE;
+        }
+        o->mat = cfg_str(cf, "material|material-file", "builtin.so");
+    }
+    if (!o->bc) {
+        if (cfg_int(cf, "boundary-conditions|use-builtin", 1) != 0) {
+            fprintf(stderr, "FATAL -- No boundary conditions.\n");
+            return EXIT_ERROR_BC_PROPS;
+        }
+        o->bc = cfg_str(cf, "boundary-conditions|boundary-conditions-file", "builtin.so");
+    }
+    o->nm = cfg_size(cf, "material|properties");
+    o->nmi = cfg_size(cf, "material|integer-properties");
+    o->nb = cfg_size(cf, "boundary-conditions|properties");
+    o->nbi = cfg_size(cf, "boundary-conditions|integer-properties");
+    if (o->nm > 64 || o->nmi > 64 || o->nb > 64 || o->nbi > 64) {
+        fprintf(stderr, "too many properties\n");
+        return EXIT_ERROR_CFG_PARSE;
+    }
+    for (size_t i = 0; i < o->nm; i++) o->mprops[i] = cfg_nfloat(cf, "material|properties", i);
+    for (size_t i = 0; i < o->nmi; i++) {
+        o->miprops_d[i] = (double)cfg_nint(cf, "material|integer-properties", i);
+    }
+    for (size_t i = 0; i < o->nb; i++) {
+        o->bprops[i] = cfg_nfloat(cf, "boundary-conditions|properties", i);
+    }
+    for (size_t i = 0; i < o->nbi; i++) {
+        o->biprops_d[i] = (double)cfg_nint(cf, "boundary-conditions|integer-properties", i);
+    }
+    const char *solver = cfg_str(cf, "solver|solver-type", "implicit");
+    if (strcmp(solver, "implicit") && strcmp(solver, "explicit-usf")
+        && strcmp(solver, "explicit-usl")) {
+        fprintf(stderr, "Invalid value for option 'solver-type': %s\n", solver);
+        return EXIT_ERROR_CFG_PARSE;
+    }
+    o->rate = cfg_float(cf, "output|sample-rate", 60.0f);
+    if (o->rate < 0) o->rate = 60.0f;
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    run_opts o;
+    memset(&o, 0, sizeof o);
+    o.rate = 60.0;
+    o.T = 1;
+    for (int a = 1; a < argc; a++) {
+        const char *k = argv[a];
+        const char *v = a + 1 < argc ? argv[a + 1] : "";
+        if (!strcmp(k, "-c")) o.cfgfile = v, a++;
+        else if (!strcmp(k, "-o")) o.outdir = v, a++;
+        else if (!strcmp(k, "-g")) o.grid = v, a++;
+        else if (!strcmp(k, "-p")) o.pfile = v, a++;
+        else if (!strcmp(k, "-u")) o.mat = v, a++;
+        else if (!strcmp(k, "-b")) o.bc = v, a++;
+        else if (!strcmp(k, "-t")) {
+            int t = atoi(v);
+            o.T = t > 0 ? (size_t)t : 1;
+            a++;
+        }
+        else if (!strcmp(k, "--dt")) o.dt = strtod(v, NULL), a++;
+        else if (!strcmp(k, "--steps")) o.steps = atoi(v), a++;
+        else if (!strcmp(k, "--mprops")) o.nm = parse_list(v, o.mprops, 64), a++;
+        else if (!strcmp(k, "--bprops")) o.nb = parse_list(v, o.bprops, 64), a++;
+        else if (!strcmp(k, "--biprops")) o.nbi = parse_list(v, o.biprops_d, 64), a++;
+        else if (!strcmp(k, "--sample-rate")) o.rate = strtod(v, NULL), a++;
+        else if (!strcmp(k, "--frames")) o.frames = v, a++;
+        else if (!strcmp(k, "--state")) o.state = v, a++;
+        else if (!strcmp(k, "--musl")) use_musl = 1;
+        else if (!strcmp(k, "--check-cfg")) o.check_cfg = 1;
+        else if (k[0] != '-' && !o.have_t_stop) {
+            o.t_stop = strtod(k, NULL);
+            o.have_t_stop = 1;
+        } else {
+            fprintf(stderr, "unknown option %s\n", k);
+            return EXIT_ERROR_CFG_PARSE;
+        }
+    }
+    cfg_file cf;
+    memset(&cf, 0, sizeof cf);
+    const int cfg_mode = o.cfgfile != NULL;
+    if (cfg_mode) {
+        if (cfg_read(&cf, o.cfgfile)) {
+            fprintf(stderr, "Fatal -- cannot parse configuration file '%s': %s\n", o.cfgfile,
+                    cf.error);
+            return EXIT_ERROR_CFG_PARSE;
+        }
+        if (!o.pfile) o.pfile = cfg_str(&cf, "input|initial-particle-file", "particles.txt");
+        if (!o.grid) o.grid = cfg_str(&cf, "input|grid-file", "grid.txt");
+    } else if (!o.grid || !o.pfile || !o.mat || !o.bc || o.dt <= 0 || o.steps <= 0) {
+        fprintf(stderr, "usage: mpm_usl_run -c SIM.cfg [-o DIR] [-p PF] [-g GF] [-u MAT.so] "
+                "[-b BC.so] [-t T] [--musl] [t_max]\n"
+                "   or: mpm_usl_run -g GRID -p PARTICLES -u MAT.so -b BC.so --dt DT --steps K "
+                "[-t T] [--mprops ..] [--bprops ..] [--biprops ..] [--musl]\n");
+        return EXIT_ERROR_CFG_PARSE;
+    }
+    if (cfg_mode && o.check_cfg) {
+        /* print the settings the run would use, without running */
+        int rc = apply_cfg(&cf, &o, NULL);
+        if (rc) return rc;
+        printf("particle-file=%s\ngrid-file=%s\nmaterial-file=%s\nbc-file=%s\n", o.pfile,
+               o.grid, o.mat, o.bc);
+        printf("dt=%.17g\nautomatic-dt=%ld\nsample-rate=%.17g\n",
+               cfg_float(&cf, "timestep|dt", 1e-6), cfg_int(&cf, "timestep|automatic-dt", 1),
+               o.rate);
+        printf("directory=%s\nuser=%s\nsolver=%s\n",
+               o.outdir ? o.outdir : cfg_str(&cf, "output|directory", "jobs/"),
+               cfg_str(&cf, "output|user", "unknown"), cfg_str(&cf, "solver|solver-type", "implicit"));
+        printf("material-properties=");
+        for (size_t i = 0; i < o.nm; i++) printf("%s%.17g", i ? "," : "", o.mprops[i]);
+        printf("\nbc-properties=");
+        for (size_t i = 0; i < o.nb; i++) printf("%s%.17g", i ? "," : "", o.bprops[i]);
+        printf("\nbc-integer-properties=");
+        for (size_t i = 0; i < o.nbi; i++) printf("%s%d", i ? "," : "", (int)o.biprops_d[i]);
+        printf("\nt_stop=%.17g\n", o.have_t_stop ? o.t_stop : 1.0);
+        cfg_release(&cf);
+        return 0;
+    }
+    int N;
+    double len;
+    particle_t *pdata;
+    size_t np;
+    if (read_grid(o.grid, &N, &len)) {
+        fprintf(stderr, "Error reading grid file.\n");
+        return EXIT_ERROR_CFG_PARSE;
+    }
+    if (read_particles(o.pfile, &pdata, &np)) {
+        fprintf(stderr, "Error reading particle file.\n");
+        return EXIT_ERROR_CFG_PARSE;
+    }
+    double h = len / (N - 1);
+    const double t_stop = cfg_mode ? (o.have_t_stop ? o.t_stop : 1.0) : (o.steps - 0.5) * o.dt;
+    job_t *job = mpm_init(N, h, pdata, np, t_stop);
+
+    char *dir = NULL;
+    FILE *info_fd = NULL, *element_fd = NULL, *state_fd = NULL, *log_fd = NULL;
+    if (cfg_mode) {
+        int rc = apply_cfg(&cf, &o, job);
+        if (rc) return rc;
+        /* timestep (src/main.c:615-636): mpm_init's automatic dt unless
+         * automatic-dt = 0 */
+        const double cfg_dt = cfg_float(&cf, "timestep|dt", 1e-6);
+        job->dt = cfg_int(&cf, "timestep|automatic-dt", 1) != 0 ? job->dt : cfg_dt;
+        /* output files (src/main.c:660-715) */
+        const char *d = o.outdir ? o.outdir : cfg_str(&cf, "output|directory", "jobs/");
+        struct stat st;
+        if (stat(d, &st) != 0 || !S_ISDIR(st.st_mode)) {
+            fprintf(stderr, "Unable to open '%s' for option 'output|directory'.\n", d);
+            return EXIT_ERROR_CFG_PARSE;
+        }
+        size_t dl = strlen(d);
+        dir = malloc(dl + 2);
+        strcpy(dir, d);
+        if (dl == 0 || dir[dl - 1] != '/') strcat(dir, "/");
+        char path[16384];
+        const char *names[5] = {"info.txt",
+                                cfg_str(&cf, "output|particle-file", "frame_particle_data.txt"),
+                                cfg_str(&cf, "output|element-file", "frame_element_data.txt"),
+                                cfg_str(&cf, "output|state-file", "state.txt"),
+                                cfg_str(&cf, "output|log-file", "job.log")};
+        FILE **fds[5] = {&info_fd, &frames_fd, &element_fd, &state_fd, &log_fd};
+        for (int k = 0; k < 5; k++) {
+            snprintf(path, sizeof path, "%s%s", dir, names[k]);
+            *fds[k] = fopen(path, "w");
+            if (!*fds[k]) {
+                fprintf(stderr, "Can't open '%s' for output.\n", path);
+                return EXIT_ERROR_CFG_PARSE;
+            }
+        }
+        job->output.directory = dir;
+        job->output.log_fd = log_fd;
+    } else {
+        job->dt = o.dt;
+        if (o.frames) frames_fd = fopen(o.frames, "w");
+    }
+    int rc = load_plugins(job, o.mat, o.bc);
+    if (rc) return rc;
+    job->material.num_fp64_props = o.nm;
+    job->material.fp64_props = o.mprops;
+    int miprops[64], biprops[64];
+    for (size_t i = 0; i < o.nmi; i++) miprops[i] = (int)o.miprops_d[i];
+    for (size_t i = 0; i < o.nbi; i++) biprops[i] = (int)o.biprops_d[i];
+    job->material.num_int_props = o.nmi;
+    job->material.int_props = miprops;
+    job->boundary.num_fp64_props = o.nb;
+    job->boundary.fp64_props = o.bprops;
+    job->boundary.num_int_props = o.nbi;
+    job->boundary.int_props = biprops;
+    job->output.sample_rate_hz = o.rate;
+
+    const size_t T = o.T;
     threadtask_t *tasks = calloc(T, sizeof(threadtask_t));
     size_t psplit = job->num_particles / T + (job->num_particles % T != 0);
     size_t nsplit = job->num_nodes / T + (job->num_nodes % T != 0);
@@ -242,7 +482,6 @@ int main(int argc, char **argv)
         return EXIT_ERROR_BC_PROPS;
     }
     job->frame = (size_t)floor(job->t * job->output.sample_rate_hz);
-    if (frames) frames_fd = fopen(frames, "w");
 
     pthread_t *th = malloc(sizeof(pthread_t) * T);
     for (size_t i = 0; i + 1 < T; i++) {
@@ -252,20 +491,19 @@ int main(int argc, char **argv)
     for (size_t i = 0; i + 1 < T; i++) {
         pthread_join(th[i], NULL);
     }
-    if (frames_fd) fclose(frames_fd);
-
-    if (state) {
-        FILE *fd = fopen(state, "w");
-        fprintf(fd, "%a %d\n", job->t, job->stepcount);
-        for (size_t i = 0; i < job->num_particles; i++) {
-            const particle_t *p = &job->particles[i];
-            fprintf(fd, "%a %a %a %a %a %a %a %a %a %a %a %a %a %d\n", p->x, p->y, p->x_t,
-                    p->y_t, p->v, p->sxx, p->sxy, p->syy, p->T[ZZ], p->state[9],
-                    p->state[10], p->ux, p->uy, job->active[i]);
-        }
-        fclose(fd);
+    if (state_fd) {
+        write_state(state_fd, job);
+    }
+    if (o.state) {
+        write_hex_state(o.state, job);
+    }
+    FILE *close_fds[5] = {info_fd, frames_fd, element_fd, state_fd, log_fd};
+    for (int k = 0; k < 5; k++) {
+        if (close_fds[k]) fclose(close_fds[k]);
     }
     printf("ran %d steps to t = %g on %zu threads\n", job->stepcount, job->t, T);
     mpm_cleanup(job);
+    free(dir);
+    cfg_release(&cf);
     return 0;
 }

@@ -16,6 +16,10 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpm_hip.so")
 BC_NONE, BC_BOX, BC_SQUARE_SILO, BC_MASK = 0, 1, 2, 3
 STORE_AUX = 0x1
 HOST_MATERIAL = 0x2
+# mpm_hip_material_kind
+MAT_G_LOCAL_MU2, MAT_G_NONLOCAL, MAT_ISOLIN = 0, 1, 2
+# mpm_hip_scheme
+SCHEME_USL, SCHEME_MUSL = 0, 1
 
 # mpm_hip_field
 FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
@@ -58,7 +62,8 @@ class Config(ctypes.Structure):
                 ("bc_int", ctypes.c_int * 4), ("bin_capacity", ctypes.c_int),
                 ("flags", ctypes.c_int), ("device", ctypes.c_int),
                 ("material_kind", ctypes.c_int), ("nl", Nonlocal),
-                ("cg_rel_tol", ctypes.c_double), ("cg_max_iter", ctypes.c_int)]
+                ("cg_rel_tol", ctypes.c_double), ("cg_max_iter", ctypes.c_int),
+                ("scheme", ctypes.c_int)]
 
 
 _lib = None
@@ -131,12 +136,16 @@ def _check(code, what):
 
 def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
                 g_mag=9.8, ramp_time=0.1, t0=0.0, bin_capacity=0, flags=0, device=0,
-                nonlocal_material=None, cg_rel_tol=1e-10, cg_max_iter=20000):
+                nonlocal_material=None, cg_rel_tol=1e-10, cg_max_iter=20000, material_kind=0,
+                scheme=0):
     """nonlocal_material: g_nonlocal constants (E, nu, mu_s, A, B, d, rho,
-    p_cap, c); selects the g_nonlocal law when given."""
+    p_cap, c); selects the g_nonlocal law when given.  material_kind:
+    MAT_G_LOCAL_MU2, MAT_G_NONLOCAL or MAT_ISOLIN."""
     lib = load()
     cfg = Config()
     lib.mpm_hip_default_config(ctypes.byref(cfg))
+    cfg.material_kind = material_kind
+    cfg.scheme = scheme
     if nonlocal_material is not None:
         cfg.material_kind = 1
         for k, val in nonlocal_material.items():

@@ -35,7 +35,7 @@ class Params(ctypes.Structure):
                 ("nl_mu_s", ctypes.c_double), ("nl_A", ctypes.c_double),
                 ("nl_B", ctypes.c_double), ("nl_d", ctypes.c_double),
                 ("nl_rho", ctypes.c_double), ("nl_p_cap", ctypes.c_double),
-                ("nl_c", ctypes.c_double)]
+                ("nl_c", ctypes.c_double), ("scheme", ctypes.c_int)]
 
 
 _libs = {}
@@ -74,6 +74,7 @@ def load(variant="parity"):
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(Params), D]
+    lib.oracle_isolin_point.argtypes = [D] + [ctypes.c_double] * 5 + [ctypes.POINTER(Params), D]
     lib.oracle_num_particle_fields.restype = ctypes.c_int
     assert lib.oracle_num_particle_fields() == len(FIELDS)
     for fn in ("oracle_global_to_local_coords",):
@@ -99,11 +100,14 @@ NONLOCAL_KEYS = ("E", "nu", "mu_s", "A", "B", "d", "rho", "p_cap", "c")
 
 def make_params(lib, N, h, dt, material=None, bc_kind=0, bc_fp64=(), bc_int=(),
                 g_mag=9.8, ramp_time=0.1, num_threads=1, log_crossings=False,
-                nonlocal_material=None):
-    """material: g_local_mu2 constants; nonlocal_material: g_nonlocal constants
-    (keys of NONLOCAL_KEYS) -- selects the g_nonlocal law when given."""
+                nonlocal_material=None, material_kind=0, scheme=0):
+    """material: g_local_mu2 constants (E and nu also serve isolin);
+    nonlocal_material: g_nonlocal constants (keys of NONLOCAL_KEYS) -- selects
+    the g_nonlocal law when given; material_kind 2 selects isolin."""
     q = Params()
     lib.oracle_default_params(ctypes.byref(q))
+    q.material_kind = material_kind
+    q.scheme = scheme     # 0 USL, 1 MUSL
     if nonlocal_material is not None:
         q.material_kind = 1
         for k, val in nonlocal_material.items():
@@ -194,5 +198,6 @@ def from_workload(w, variant="parity", num_threads=1, log_crossings=False, parti
                     bc_int=w.get("bc_int", ()), g_mag=w.get("g_mag", 9.8),
                     ramp_time=w.get("ramp_time", 0.1), num_threads=num_threads,
                     log_crossings=log_crossings,
-                    nonlocal_material=w.get("nonlocal_material"))
+                    nonlocal_material=w.get("nonlocal_material"),
+                    material_kind=w.get("material_kind", 0), scheme=w.get("scheme", 0))
     return Oracle(q, particles if particles is not None else w["particles"], variant=variant)

@@ -13,6 +13,7 @@
  *   mpm_init                       libmpm/process_usl.c:68-280
  *   WHICH_ELEMENT4                 libmpm/process_usl.c:43-44
  *   explicit_mpm_step_usl_threaded libmpm/process_usl.c:465-592
+ *   explicit_mpm_step_musl_threaded libmpm/process_usl.c:284-462
  *   move_grid_split                libmpm/process_usl.c:596-625
  *   create_particle_to_element_map libmpm/process_usl.c:631-674
  *   find_filled_elements           libmpm/process_usl.c:677-723
@@ -91,12 +92,15 @@ typedef struct oracle_params {
     int bc_int[4];
     int log_crossings;     /* write the reference's per-crossing log lines */
     const char *log_path;  /* NULL: /dev/null when logging */
-    /* material: 0 g_local_mu2, 1 g_nonlocal (materialsrc/g_nonlocal.c) */
+    /* material: 0 g_local_mu2, 1 g_nonlocal (materialsrc/g_nonlocal.c),
+     * 2 isolin (materialsrc/isolin.c; E and nu from the fields above) */
     int material_kind;
     /* g_nonlocal constants (g_nonlocal.c:31-47, 141, 313):
      * E (EMOD 1e8), nu (NUMOD 0.3), mu_s (0.3819), A (0.48), B (0.9377*20),
      * grains_d (0.01*5), grains_rho (2450), p_cap (1e-2), c (1e-2) */
     double nl_E, nl_nu, nl_mu_s, nl_A, nl_B, nl_d, nl_rho, nl_p_cap, nl_c;
+    /* step: 0 explicit_mpm_step_usl_threaded, 1 explicit_mpm_step_musl_threaded */
+    int scheme;
 } oracle_params_t;
 
 typedef struct oracle_job {
@@ -408,6 +412,38 @@ static void g_local_mu2_stress_threaded(threadtask_t *task)
         }
         oracle_g_local_mu2_point(&job->particles[i], job->dt, oj->G, oj->K,
             q->mu_s, q->mu_2, q->I_0, q->grains_d, q->grains_rho, q->rho_crit);
+    }
+}
+
+/* calculate_stress_threaded of materialsrc/isolin.c (72-116): isotropic
+ * linear elasticity with the Jaumann rate on sxx, sxy, syy; G and K as
+ * material_init (isolin.c:30-58) derives them. */
+static void isolin_point(particle_t *p, double dt, double G, double K)
+{
+    const double lambda = K - 2.0 * G / 3.0;
+    const double trD = p->exx_t + p->eyy_t;
+    double dsjxx = lambda * trD + 2.0 * G * p->exx_t;
+    double dsjxy = 2.0 * G * p->exy_t;
+    double dsjyy = lambda * trD + 2.0 * G * p->eyy_t;
+    dsjxx += 2 * p->wxy_t * p->sxy;
+    dsjxy -= p->wxy_t * (p->sxx - p->syy);
+    dsjyy -= 2 * p->wxy_t * p->sxy;
+    p->sxx += dt * dsjxx;
+    p->sxy += dt * dsjxy;
+    p->syy += dt * dsjyy;
+}
+
+static void isolin_stress_threaded(threadtask_t *task)
+{
+    oracle_job_t *oj = OJ(task->job);
+    job_t *job = task->job;
+    size_t p_start = task->offset;
+    size_t p_stop = task->offset + task->blocksize;
+    for (size_t i = p_start; i < p_stop; i++) {
+        if (job->active[i] == 0) {
+            continue;
+        }
+        isolin_point(&job->particles[i], job->dt, oj->G, oj->K);
     }
 }
 
@@ -1266,19 +1302,8 @@ static void update_particle_densities_split(job_t *job, size_t p_start, size_t p
     }
 }
 
-static void explicit_mpm_step_usl_threaded(threadtask_t *task)
+static void clear_nodes(job_t *job, size_t n_start, size_t n_stop)
 {
-    job_t *job = task->job;
-    oracle_job_t *oj = OJ(job);
-    int rc;
-    size_t p_start = task->offset;
-    size_t p_stop = task->offset + task->blocksize;
-    size_t n_start = task->n_offset;
-    size_t n_stop = task->n_offset + task->n_blocksize;
-    size_t e_start = task->e_offset;
-    size_t e_stop = task->e_offset + task->e_blocksize;
-
-    pthread_barrier_wait(job->serialize_barrier);
     for (size_t i = n_start; i < n_stop; i++) {
         node_t *nd = &job->nodes[i];
         nd->m = 0;
@@ -1295,6 +1320,41 @@ static void explicit_mpm_step_usl_threaded(threadtask_t *task)
         nd->ux = 0;
         nd->uy = 0;
     }
+}
+
+/* The material's calculate_stress_threaded (process_usl.c:589). */
+static void stress_update(threadtask_t *task)
+{
+    job_t *job = task->job;
+    oracle_job_t *oj = OJ(job);
+    if (oj->prm.material_kind == 1) {
+        /* g_nonlocal runs on one thread (g_nonlocal_mu2.c:113-123); it reads
+         * every particle's strain rate, so the others must be done */
+        pthread_barrier_wait(job->serialize_barrier);
+        if (task->id == 0) {
+            g_nonlocal_stress(oj);
+        }
+    } else if (oj->prm.material_kind == 2) {
+        isolin_stress_threaded(task);
+    } else {
+        g_local_mu2_stress_threaded(task);
+    }
+}
+
+static void explicit_mpm_step_usl_threaded(threadtask_t *task)
+{
+    job_t *job = task->job;
+    oracle_job_t *oj = OJ(job);
+    int rc;
+    size_t p_start = task->offset;
+    size_t p_stop = task->offset + task->blocksize;
+    size_t n_start = task->n_offset;
+    size_t n_stop = task->n_offset + task->n_blocksize;
+    size_t e_start = task->e_offset;
+    size_t e_stop = task->e_offset + task->e_blocksize;
+
+    pthread_barrier_wait(job->serialize_barrier);
+    clear_nodes(job, n_start, n_stop);
     for (size_t i = e_start; i < e_stop; i++) {
         job->elements[i].filled = 0;
         job->elements[i].n = 0;
@@ -1330,16 +1390,85 @@ static void explicit_mpm_step_usl_threaded(threadtask_t *task)
     move_particles_explicit_usl_split(job, p_start, p_stop);
     calculate_strainrate_split(job, p_start, p_stop);
     update_particle_densities_split(job, p_start, p_stop);
-    if (oj->prm.material_kind == 1) {
-        /* g_nonlocal runs on one thread (g_nonlocal_mu2.c:113-123); it reads
-         * every particle's strain rate, so the others must be done */
-        pthread_barrier_wait(job->serialize_barrier);
-        if (task->id == 0) {
-            g_nonlocal_stress(oj);
-        }
-    } else {
-        g_local_mu2_stress_threaded(task);
+    stress_update(task);
+}
+
+/* explicit_mpm_step_musl_threaded (process_usl.c:284-462): the USL step up
+ * to the particle update, then the particle momentum is mapped to the grid
+ * again with the step's shape functions, the node velocities recomputed
+ * (process_usl.c:429-440; bc_momentum then zeroes the node momenta, not the
+ * velocities the strain rate reads, 441-445), and the strain rate, volume
+ * and stress follow from those velocities. */
+static void explicit_mpm_step_musl_threaded(threadtask_t *task)
+{
+    job_t *job = task->job;
+    oracle_job_t *oj = OJ(job);
+    int rc;
+    size_t p_start = task->offset;
+    size_t p_stop = task->offset + task->blocksize;
+    size_t n_start = task->n_offset;
+    size_t n_stop = task->n_offset + task->n_blocksize;
+    size_t e_start = task->e_offset;
+    size_t e_stop = task->e_offset + task->e_blocksize;
+
+    pthread_barrier_wait(job->serialize_barrier);
+    clear_nodes(job, n_start, n_stop);
+    for (size_t i = e_start; i < e_stop; i++) {
+        job->elements[i].filled = 0;
+        job->elements[i].n = 0;
+        job->elements[i].m = 0;
     }
+    create_particle_to_element_map_threaded(task);
+    calculate_shapefunctions_split(job, p_start, p_stop);
+
+    rc = pthread_barrier_wait(job->serialize_barrier);
+    if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
+        job->t += job->dt;
+        int flag = 0;
+        for (size_t i = 0; i < job->num_threads; i++) {
+            flag += job->update_elementlists[i];
+        }
+        if (flag != 0) {
+            find_filled_elements(job);
+        }
+        bc_time_varying(oj);
+    }
+    pthread_barrier_wait(job->serialize_barrier);
+    map_to_grid_explicit_split(job, task->id);
+
+    rc = pthread_barrier_wait(job->serialize_barrier);
+    if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
+        bc_momentum(oj);
+        bc_force(oj);
+    }
+    pthread_barrier_wait(job->serialize_barrier);
+    move_grid_split(job, n_start, n_stop);
+    pthread_barrier_wait(job->serialize_barrier);
+    move_particles_explicit_usl_split(job, p_start, p_stop);
+
+    pthread_barrier_wait(job->serialize_barrier);
+    clear_nodes(job, n_start, n_stop);
+    pthread_barrier_wait(job->serialize_barrier);
+    map_to_grid_explicit_split(job, task->id);
+    pthread_barrier_wait(job->serialize_barrier);
+    for (size_t i = n_start; i < n_stop; i++) {
+        node_t *nd = &job->nodes[i];
+        if (nd->m > TOL) {
+            nd->x_t = nd->mx_t / nd->m;
+            nd->y_t = nd->my_t / nd->m;
+        } else {
+            nd->x_t = 0;
+            nd->y_t = 0;
+        }
+    }
+    rc = pthread_barrier_wait(job->serialize_barrier);
+    if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
+        bc_momentum(oj);
+    }
+    pthread_barrier_wait(job->serialize_barrier);
+    calculate_strainrate_split(job, p_start, p_stop);
+    update_particle_densities_split(job, p_start, p_stop);
+    stress_update(task);
 }
 
 /* mpm_run_until (src/main.c:934-982) with an exact step count in place of
@@ -1350,7 +1479,11 @@ static void *run_steps(void *arg)
     job_t *job = task->job;
     oracle_job_t *oj = OJ(job);
     for (int s = 0; s < oj->steps_to_run; s++) {
-        explicit_mpm_step_usl_threaded(task);
+        if (OJ(job)->prm.scheme == 1) {
+            explicit_mpm_step_musl_threaded(task);
+        } else {
+            explicit_mpm_step_usl_threaded(task);
+        }
         int rc = pthread_barrier_wait(job->serialize_barrier);
         if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
             job->stepcount++;
@@ -1624,6 +1757,27 @@ double oracle_exp(double x)
  * applies the given velocity gradient (exx, exy, eyy, wxy) for one step to
  * the state (T[4] = xx, xy, yy, zz; m, v; gammap) and returns the result in
  * out[6] = {Txx, Txy, Tyy, Tzz, gammap, gammadotp}. */
+/* One isolin update: s = {sxx, sxy, syy} -> out (test hook). */
+void oracle_isolin_point(const double *sin, double exx, double exy, double eyy,
+    double wxy, double dt, const oracle_params_t *q, double *out)
+{
+    particle_t p;
+    memset(&p, 0, sizeof p);
+    p.sxx = sin[0];
+    p.sxy = sin[1];
+    p.syy = sin[2];
+    p.exx_t = exx;
+    p.exy_t = exy;
+    p.eyy_t = eyy;
+    p.wxy_t = wxy;
+    double G = q->E / (2.0 * (1.0 + q->nu));
+    double K = q->E / (3.0 * (1.0 - 2 * q->nu));
+    isolin_point(&p, dt, G, K);
+    out[0] = p.sxx;
+    out[1] = p.sxy;
+    out[2] = p.syy;
+}
+
 void oracle_stress_point(const double *Tin, double m, double v, double gammap,
     double exx, double exy, double eyy, double wxy, double dt,
     const oracle_params_t *q, double *out)

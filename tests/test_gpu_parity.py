@@ -35,7 +35,8 @@ def make_sim(w, particles=None, flags=0, cap=0, bc_kind=None):
                             bc_kind=w.get("bc_kind", 0) if bc_kind is None else bc_kind,
                             bc_fp64=w.get("bc_fp64", ()), bc_int=w.get("bc_int", ()),
                             g_mag=w.get("g_mag", 9.8), ramp_time=w.get("ramp_time", 0.1),
-                            bin_capacity=cap, flags=flags)
+                            bin_capacity=cap, flags=flags,
+                            material_kind=w.get("material_kind", 0))
     return pympm.Sim(cfg, particles if particles is not None else w["particles"])
 
 

@@ -1,0 +1,264 @@
+"""The MUSL step (libmpm/process_usl.c:284-462) and the reference-style
+driver (configuration file, frame and state files).
+
+CPU: the configuration reader on files in the reference's libconfuse syntax
+(sections, lists, comments, environment references) and its errors.
+GPU: MUSL bitwise against the oracle's restatement through the C ABI and
+the drop-in; a run driven by a .cfg whose frame file equals, byte for byte,
+the reference writer's format applied to the oracle's states, and whose
+state file carries the oracle's values.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pympm
+import pyoracle
+import workloads
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(os.path.dirname(HERE), "mpm-2d_amd")
+RUN = os.path.join(PKG, "bin", "mpm_usl_run")
+
+CFG = """timestep
+{{
+    dt-max = 1e-2
+    dt-min = 1e-10
+    dt = {dt!r}   // comment after a value
+    automatic-dt = 0
+    allow-dt-increase = 0
+}}
+
+solver
+{{
+    solver-type = explicit-usl
+}}
+
+/* a block
+   comment */
+material
+{{
+    material-file = "{mat}"
+    use-builtin = 0
+    properties = {{{mprops}}}
+        # properties are Young's modulus and Poisson's ratio
+    integer-properties = {{ }}
+}}
+
+boundary-conditions
+{{
+    boundary-conditions-file = "{bc}"
+    use-builtin = 0
+    properties = {{{bprops}}}
+    integer-properties = {{ {biprops} }}
+}}
+
+input
+{{
+    initial-particle-file = "{pfile}"
+    grid-file = "{gfile}"
+}}
+
+output
+{{
+    directory = "{outdir}"
+    user = ${{MPM_TEST_UNSET_VAR:-unknown}}
+    sample-rate = {rate!r}
+}}
+"""
+
+
+def write_cfg(tmp_path, w, rate=20000.0, plugin_dir="lib"):
+    g, p = tmp_path / "grid.txt", tmp_path / "particles.txt"
+    workloads.write_reference_inputs(w, g, p)
+    out = tmp_path / "out"
+    out.mkdir(exist_ok=True)
+    m = w["material"]
+    text = CFG.format(dt=w["dt"], mat=os.path.join(PKG, plugin_dir, "g_local_mu2.so"),
+                      mprops=", ".join(repr(float(m[k])) for k in
+                                       ("E", "nu", "mu_s", "mu_2", "I_0", "grains_d",
+                                        "grains_rho", "rho_crit")),
+                      bc=os.path.join(PKG, plugin_dir, "box.so"),
+                      bprops=", ".join(repr(float(v)) for v in w["bc_fp64"]),
+                      biprops=", ".join(str(int(v)) for v in w["bc_int"]),
+                      pfile=p, gfile=g, outdir=out, rate=rate)
+    path = tmp_path / "simulation.cfg"
+    path.write_text(text)
+    return path, out
+
+
+def check_cfg(args):
+    return subprocess.run([RUN] + [str(a) for a in args] + ["--check-cfg"],
+                          capture_output=True, text=True, timeout=60)
+
+
+# ---------------------------------------------------------------------------
+# CPU
+
+def test_cfg_reader_reference_syntax(tmp_path):
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    path, out = write_cfg(tmp_path, w)
+    r = check_cfg(["-c", path, "0.25"])
+    assert r.returncode == 0, r.stderr
+    got = dict(line.split("=", 1) for line in r.stdout.strip().split("\n"))
+    assert float(got["dt"]) == w["dt"]
+    assert got["automatic-dt"] == "0"
+    assert float(got["sample-rate"]) == 20000.0
+    assert got["solver"] == "explicit-usl"
+    assert got["user"] == "unknown"
+    assert got["directory"] == str(out)
+    assert [float(v) for v in got["material-properties"].split(",")][:2] == [1e6, 0.3]
+    assert [int(v) for v in got["bc-integer-properties"].split(",")] == [1, 3, 1, 0]
+    assert float(got["t_stop"]) == 0.25
+    # command-line overrides, as src/main.c:333-361 applies them
+    r = check_cfg(["-c", path, "-p", "other.txt", "-u", "x.so", "-o", "/tmp"])
+    got = dict(line.split("=", 1) for line in r.stdout.strip().split("\n"))
+    assert got["particle-file"] == "other.txt" and got["material-file"] == "x.so"
+    assert got["directory"] == "/tmp" and float(got["t_stop"]) == 1.0
+
+
+def test_cfg_reader_errors(tmp_path):
+    bad = tmp_path / "bad.cfg"
+    bad.write_text("material { properties = {1, 2 \n")
+    assert check_cfg(["-c", bad]).returncode == 0x20          # EXIT_ERROR_CFG_PARSE
+    builtin = tmp_path / "builtin.cfg"
+    builtin.write_text("material { use-builtin = 1 }\n")
+    assert check_cfg(["-c", builtin]).returncode == 0x21      # EXIT_ERROR_MATERIAL_FILE
+    solver = tmp_path / "solver.cfg"
+    solver.write_text("solver { solver-type = explicit-xyz }\nmaterial { use-builtin = 0 }\n"
+                      "boundary-conditions { use-builtin = 0 }\n")
+    assert check_cfg(["-c", solver]).returncode == 0x20
+
+
+# ---------------------------------------------------------------------------
+# GPU: MUSL
+
+PF = ["x", "y", "vx", "vy", "v", "txx", "txy", "tyy", "tzz", "gammap", "gammadotp", "ux",
+      "uy"]
+
+MUSL_CASES = {
+    "box": lambda: workloads.small_box(cells=16, bx=5, by=9),
+    "silo_64": lambda: workloads.silo(seed=5, cells=64),
+    "box_isolin": lambda: dict(workloads.small_box(cells=16, bx=5, by=9), material_kind=2),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(MUSL_CASES))
+def test_gpu_musl_bitwise(case):
+    w = dict(MUSL_CASES[case](), scheme=1)
+    steps = 100
+    print(f"MUSL {case}: np={len(w['particles']['x'])} steps={steps}", flush=True)
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
+                            g_mag=w["g_mag"], ramp_time=w["ramp_time"],
+                            material_kind=w.get("material_kind", 0), scheme=pympm.SCHEME_MUSL)
+    sim = pympm.Sim(cfg, w["particles"])
+    sim.step(steps)
+    o = pyoracle.from_workload(w, num_threads=3)
+    o.run(steps)
+    ref = o.particles()
+    if w.get("material_kind") == 2:
+        ref = dict(ref, txx=ref["sxx"], txy=ref["sxy"], tyy=ref["syy"])
+        fields = ["x", "y", "vx", "vy", "v", "txx", "txy", "tyy", "ux", "uy"]
+    else:
+        fields = PF
+    for k in fields:
+        assert np.array_equal(sim.get(k), ref[k]), k
+    assert np.array_equal(sim.active(), ref["active"])
+    # MUSL differs from USL on the same inputs
+    u = pyoracle.from_workload(dict(w, scheme=0), num_threads=2)
+    u.run(steps)
+    assert not np.array_equal(u.particles()["x"], ref["x"])
+
+
+@pytest.mark.gpu
+def test_dropin_musl(tmp_path):
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    steps = 40
+    g, p = tmp_path / "grid.txt", tmp_path / "particles.txt"
+    workloads.write_reference_inputs(w, g, p)
+    m = w["material"]
+    cmd = [RUN, "-g", g, "-p", p, "-u", os.path.join(PKG, "lib", "g_local_mu2.so"),
+           "-b", os.path.join(PKG, "lib", "box.so"), "-t", "3", "--dt", repr(w["dt"]),
+           "--steps", str(steps), "--musl",
+           "--mprops", ",".join(repr(float(m[k])) for k in
+                                ("E", "nu", "mu_s", "mu_2", "I_0", "grains_d", "grains_rho",
+                                 "rho_crit")),
+           "--bprops", ",".join(repr(float(v)) for v in w["bc_fp64"]),
+           "--biprops", ",".join(str(int(v)) for v in w["bc_int"]),
+           "--state", tmp_path / "state.txt"]
+    print("case drop-in MUSL", flush=True)
+    out = subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    lines = open(tmp_path / "state.txt").read().split("\n")
+    rows = [ln.split() for ln in lines[1:] if ln.strip()]
+    cols = ["x", "y", "vx", "vy", "v", "txx", "txy", "tyy", "tzz", "gammap", "gammadotp", "ux",
+            "uy"]
+    o = pyoracle.from_workload(dict(w, scheme=1), num_threads=2)
+    o.run(steps)
+    r = o.particles()
+    for i, k in enumerate(cols):
+        got = np.array([float.fromhex(row[i]) for row in rows])
+        assert np.array_equal(got, r[k]), k
+
+
+# ---------------------------------------------------------------------------
+# GPU: a run driven by a configuration file
+
+def frame_text(frame, t, st):
+    """src/writer.c:169-201 (write_frame / write_particle) applied to one
+    oracle state; %lg as Python's %g."""
+    n = len(st["x"])
+    out = ["%d %s %d" % (frame, "%g" % t, n)]
+    for i in range(n):
+        vals = [st["m"][i], st["v"][i], st["x"][i], st["y"][i], st["vx"][i], st["vy"][i],
+                st["txx"][i], st["txy"][i], st["tyy"][i], st["ux"][i], st["uy"][i],
+                st["gammap"][i], 0.0, st["gammadotp"][i], float(st["active"][i])]
+        out.append(" ".join("%g" % v for v in vals) + "".join(" 0" for _ in range(8)))
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.gpu
+def test_cfg_run_frames_and_state(tmp_path):
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    steps, rate = 30, 20000.0
+    path, outdir = write_cfg(tmp_path, w, rate=rate)
+    t_stop = (steps - 0.5) * w["dt"]
+    print("case cfg run", flush=True)
+    r = subprocess.run([RUN, "-c", str(path), "-t", "3", repr(t_stop)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    # expected frames: the reference loop writes after a step whenever
+    # t >= frame / rate (src/main.c:955-959), frame starting at floor(t0 rate)
+    o = pyoracle.from_workload(w, num_threads=2)
+    expected, frame = [], 0
+    for _ in range(steps):
+        o.run(1)
+        t = o.time()
+        if t >= frame / rate:
+            expected.append(frame_text(frame, t, o.particles()))
+            frame += 1
+    got = (outdir / "frame_particle_data.txt").read_text()
+    assert got == "".join(expected)
+    # state file (src/writer.c:266-380): header and per-particle values
+    st = o.particles()
+    lines = (outdir / "state.txt").read_text().split("\n")
+    n = len(st["x"])
+    assert lines[0].split()[0] == "%g" % o.time()
+    assert lines[1].split() == [str(n), str(w["N"] ** 2), str((w["N"] - 1) ** 2)]
+    per = 12 + 1 + 11 + 1      # lines per particle: fields, DEPVAR, state[11], active
+    for i in range(0, n, 7):
+        blk = lines[3 + i * per: 3 + (i + 1) * per]
+        assert blk[0].split() == ["%g" % st["x"][i], "%g" % st["y"][i]]
+        assert blk[1].split() == ["%g" % st["xl"][i], "%g" % st["yl"][i]]
+        assert blk[2].split() == ["%g" % st["vx"][i], "%g" % st["vy"][i]]
+        assert blk[6].split() == ["%g" % st[k][i] for k in ("txx", "txy", "tyy")]
+        assert blk[7].split() == ["%g" % st[k][i] for k in ("exx", "exy", "eyy", "wxy")]
+        assert blk[10].split() == ["%g" % st["ux"][i], "%g" % st["uy"][i]]
+        assert blk[12] == "11"
+        assert blk[13 + 9] == "%g" % st["gammap"][i]
+        assert blk[13 + 10] == "%g" % st["gammadotp"][i]
+        assert blk[24] == str(int(st["active"][i]))
