@@ -107,10 +107,15 @@ enum mpm_hip_material_kind {
     MPM_MAT_G_NONLOCAL = 1,    /* materialsrc/g_nonlocal.c: per-step FEM
                                   fluidity solve, here Jacobi-preconditioned
                                   conjugate gradients on the device */
-    MPM_MAT_ISOLIN = 2         /* materialsrc/isolin.c: isotropic linear
+    MPM_MAT_ISOLIN = 2,        /* materialsrc/isolin.c: isotropic linear
                                   elasticity with the Jaumann rate, on the
                                   in-plane stress (material.E, material.nu;
                                   MPM_F_TXX/TXY/TYY hold sxx/sxy/syy) */
+    MPM_MAT_DP_RI_MT = 3       /* materialsrc/dp_ri_mt.c: rate-independent
+                                  Drucker-Prager return map (material.E, nu,
+                                  mu_s); MPM_F_TXX/TXY/TYY hold sxx/sxy/syy,
+                                  MPM_F_TZZ szz (state[1]), MPM_F_GAMMAP and
+                                  MPM_F_GAMMADOTP state[9], state[10] */
 };
 
 typedef struct mpm_hip_config {

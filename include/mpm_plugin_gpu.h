@@ -32,14 +32,15 @@ extern "C" {
 
 enum mpm_gpu_material_kind {
     MPM_GPU_MATERIAL_G_LOCAL_MU2 = 1,  /* materialsrc/g_local_mu2.c */
-    MPM_GPU_MATERIAL_ISOLIN = 2        /* materialsrc/isolin.c */
+    MPM_GPU_MATERIAL_ISOLIN = 2,       /* materialsrc/isolin.c */
+    MPM_GPU_MATERIAL_DP_RI_MT = 3      /* materialsrc/dp_ri_mt.c */
 };
 
 typedef struct mpm_material_gpu_desc {
     int abi;        /* MPM_PLUGIN_GPU_ABI */
     int kind;       /* mpm_gpu_material_kind */
     /* g_local_mu2: E, nu, mu_s, mu_2, I_0, grains_d, grains_rho, rho_crit;
-     * isolin: E, nu */
+     * isolin: E, nu; dp_ri_mt: E, nu, mu_s */
     double params[8];
 } mpm_material_gpu_desc;
 

@@ -61,7 +61,7 @@ constexpr int kG2PThreads = 256;
 
 enum : int { kStatusBinOverflow = 1 };
 enum : int { kModeStoreAux = 1, kModeHostMaterial = 2, kModeNonlocal = 4, kModeIsolin = 8,
-              kModeMuslMove = 16, kModeMuslFinish = 32 };
+              kModeMuslMove = 16, kModeMuslFinish = 32, kModeDpRiMt = 64 };
 
 struct DevState {
     double t;           // job->t
@@ -870,6 +870,53 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         P.txx[p] = sxx + dt * dsjxx;
         P.txy[p] = sxy + dt * dsjxy;
         P.tyy[p] = syy + dt * dsjyy;
+    } else if (mode & kModeDpRiMt) {
+        // dp_ri_mt calculate_stress_threaded (materialsrc/dp_ri_mt.c:133-205)
+        // on sxx, sxy, syy (Txx, Txy, Tyy arrays) and szz = state[1] (Tzz)
+        const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p], szz = P.tzz[p];
+        const double trD = exx + eyy;
+        double dsjxx = mt.lambda * trD + mt.two_G * exx;
+        double dsjxy = mt.two_G * exy;
+        double dsjyy = mt.lambda * trD + mt.two_G * eyy;
+        dsjxx += 2 * wxy * sxy;
+        dsjxy -= wxy * (sxx - syy);
+        dsjyy -= 2 * wxy * sxy;
+        const double sxx_tr = sxx + dt * dsjxx;
+        const double sxy_tr = sxy + dt * dsjxy;
+        const double syy_tr = syy + dt * dsjyy;
+        const double szz_tr = szz + dt * mt.lambda * trD;
+        const double p_tr = -(sxx_tr + syy_tr + szz_tr) / 3.0;
+        const double t0xx = sxx_tr + p_tr, t0xy = sxy_tr, t0yy = syy_tr + p_tr;
+        const double t0zz = szz_tr + p_tr;
+        const double tau_tr = sqrt(0.5 * (t0xx * t0xx + 2 * t0xy * t0xy + t0yy * t0yy
+                                          + t0zz * t0zz));
+        double nup_tau;
+        if ((mp / v) < mt.rho_crit || p_tr <= 0.0) {
+            nup_tau = tau_tr / (mt.G * dt);
+            P.txx[p] = 0.0;
+            P.txy[p] = 0.0;
+            P.tyy[p] = 0.0;
+            P.tzz[p] = 0.0;
+        } else if (p_tr > 0.0) {
+            const double S0 = mt.mu_s * p_tr;
+            double tau_tau, scale;
+            if (tau_tr <= S0) {
+                tau_tau = tau_tr;
+                scale = 1.0;
+            } else {
+                tau_tau = S0;
+                scale = tau_tau / tau_tr;
+            }
+            nup_tau = ((tau_tr - tau_tau) / mt.G) / dt;
+            P.txx[p] = scale * t0xx - p_tr;
+            P.txy[p] = scale * t0xy;
+            P.tyy[p] = scale * t0yy - p_tr;
+            P.tzz[p] = scale * t0zz - p_tr;
+        } else {
+            nup_tau = 0.0;   // NaN pressure: stress left unchanged
+        }
+        P.gp[p] = P.gp[p] + nup_tau * dt;
+        P.gdp[p] = nup_tau;
     } else if (!(mode & kModeHostMaterial)) {
         double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
         double gp = P.gp[p], gdp;
@@ -1962,6 +2009,9 @@ static int launch_g2p(mpm_hip_ctx *ctx)
     if (ctx->cfg.material_kind == MPM_MAT_ISOLIN && !(mode & kModeHostMaterial)) {
         mode |= kModeIsolin;
     }
+    if (ctx->cfg.material_kind == MPM_MAT_DP_RI_MT && !(mode & kModeHostMaterial)) {
+        mode |= kModeDpRiMt;
+    }
     if (ctx->cfg.scheme == MPM_SCHEME_MUSL) {
         // particle update; second mapping of the momentum; strain rate,
         // volume, stress and rebinning from the re-mapped velocities
@@ -2196,7 +2246,7 @@ static int params_from_config(const mpm_hip_config *cfg, int cap, GridParams &g,
     mt.nl_p_cap = cfg->nonlocal.p_cap;
     mt.nl_c = cfg->nonlocal.c;
     if (cfg->material_kind != MPM_MAT_G_LOCAL_MU2 && cfg->material_kind != MPM_MAT_G_NONLOCAL
-        && cfg->material_kind != MPM_MAT_ISOLIN) {
+        && cfg->material_kind != MPM_MAT_ISOLIN && cfg->material_kind != MPM_MAT_DP_RI_MT) {
         return MPM_HIP_ERR_INVALID;
     }
     return MPM_HIP_OK;

@@ -48,7 +48,7 @@ typedef struct gpu_job {
     mpm_hip_ctx *ctx;
     int ready;
     int host_material;
-    int isolin;             /* device isolin: the stress lives in sxx, sxy, syy */
+    int s_law;              /* device law on s: 1 isolin, 2 dp_ri_mt (szz in state[1]) */
     int scheme;             /* MPM_SCHEME_USL or MPM_SCHEME_MUSL, fixed by the first step */
     int host_bc;
     int aux_on;
@@ -508,7 +508,15 @@ static void gpu_setup(job_t *job)
         cfg.material.E = md.params[0];
         cfg.material.nu = md.params[1];
         G.host_material = 0;
-        G.isolin = 1;
+        G.s_law = 1;
+    } else if (have && md.kind == MPM_GPU_MATERIAL_DP_RI_MT) {
+        cfg.material_kind = MPM_MAT_DP_RI_MT;
+        cfg.material.E = md.params[0];
+        cfg.material.nu = md.params[1];
+        cfg.material.mu_s = md.params[2];
+        cfg.material.rho_crit = 1485.0f;   /* dp_ri_mt.c:165 */
+        G.host_material = 0;
+        G.s_law = 2;
     } else if (have && md.kind == MPM_GPU_MATERIAL_G_LOCAL_MU2) {
         cfg.material.E = md.params[0];
         cfg.material.nu = md.params[1];
@@ -563,7 +571,7 @@ static void gpu_setup(job_t *job)
         vy[i] = p->y_t;
         /* the device law's stress: isolin and host laws act on s, g_local_mu2
          * on T (which its material_init set from s) */
-        const int on_s = G.host_material || G.isolin;
+        const int on_s = G.host_material || G.s_law;
         sxx[i] = on_s ? p->sxx : p->T[XX];
         sxy[i] = on_s ? p->sxy : p->T[XY];
         syy[i] = on_s ? p->syy : p->T[YY];
@@ -645,11 +653,18 @@ static void download_kinematics(void)
 static void download_stress(void)
 {
     job_t *job = G.job;
-    if (G.isolin) {
-        /* isolin keeps its stress in s only (materialsrc/isolin.c:107-109) */
+    if (G.s_law) {
+        /* isolin and dp_ri_mt keep their stress in s (isolin.c:107-109,
+         * dp_ri_mt.c:167-194); dp_ri_mt also szz and the plastic strain in
+         * state[1], state[9], state[10] */
         get_into(MPM_F_TXX, POFF(sxx));
         get_into(MPM_F_TXY, POFF(sxy));
         get_into(MPM_F_TYY, POFF(syy));
+        if (G.s_law == 2) {
+            get_into(MPM_F_TZZ, SOFF(1));
+            get_into(MPM_F_GAMMAP, SOFF(9));
+            get_into(MPM_F_GAMMADOTP, SOFF(10));
+        }
         return;
     }
     get_into(MPM_F_TXX, TOFF(XX));

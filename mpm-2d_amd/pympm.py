@@ -17,7 +17,7 @@ BC_NONE, BC_BOX, BC_SQUARE_SILO, BC_MASK = 0, 1, 2, 3
 STORE_AUX = 0x1
 HOST_MATERIAL = 0x2
 # mpm_hip_material_kind
-MAT_G_LOCAL_MU2, MAT_G_NONLOCAL, MAT_ISOLIN = 0, 1, 2
+MAT_G_LOCAL_MU2, MAT_G_NONLOCAL, MAT_ISOLIN, MAT_DP_RI_MT = 0, 1, 2, 3
 # mpm_hip_scheme
 SCHEME_USL, SCHEME_MUSL = 0, 1
 

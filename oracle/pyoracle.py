@@ -16,7 +16,7 @@ REF_PIECES = os.path.join(HERE, "_ref", "libmpm_ref_pieces.so")
 FIELDS = ["x", "y", "vx", "vy", "m", "v", "txx", "txy", "tyy", "tzz",
           "gammap", "gammadotp", "ux", "uy", "exx", "exy", "eyy", "wxy",
           "ax", "ay", "xl", "yl", "sxx", "sxy", "syy", "txz", "tyx", "tyz",
-          "tzx", "tzy", "by", "gf", "gf_bulk", "xisq_inv"]
+          "tzx", "tzy", "by", "gf", "gf_bulk", "xisq_inv", "state1"]
 NODE_FIELDS = ["m", "mx", "my", "fx", "fy", "vx", "vy", "ax", "ay"]
 
 
@@ -75,6 +75,7 @@ def load(variant="parity"):
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(Params), D]
     lib.oracle_isolin_point.argtypes = [D] + [ctypes.c_double] * 5 + [ctypes.POINTER(Params), D]
+    lib.oracle_dp_ri_mt_point.argtypes = [D] + [ctypes.c_double] * 7 + [ctypes.POINTER(Params), D]
     lib.oracle_num_particle_fields.restype = ctypes.c_int
     assert lib.oracle_num_particle_fields() == len(FIELDS)
     for fn in ("oracle_global_to_local_coords",):

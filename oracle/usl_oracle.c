@@ -93,7 +93,8 @@ typedef struct oracle_params {
     int log_crossings;     /* write the reference's per-crossing log lines */
     const char *log_path;  /* NULL: /dev/null when logging */
     /* material: 0 g_local_mu2, 1 g_nonlocal (materialsrc/g_nonlocal.c),
-     * 2 isolin (materialsrc/isolin.c; E and nu from the fields above) */
+     * 2 isolin (materialsrc/isolin.c; E and nu from the fields above),
+     * 3 dp_ri_mt (materialsrc/dp_ri_mt.c; E, nu, mu_s from the fields above) */
     int material_kind;
     /* g_nonlocal constants (g_nonlocal.c:31-47, 141, 313):
      * E (EMOD 1e8), nu (NUMOD 0.3), mu_s (0.3819), A (0.48), B (0.9377*20),
@@ -431,6 +432,72 @@ static void isolin_point(particle_t *p, double dt, double G, double K)
     p->sxx += dt * dsjxx;
     p->sxy += dt * dsjxy;
     p->syy += dt * dsjyy;
+}
+
+/* One particle of dp_ri_mt's calculate_stress_threaded
+ * (materialsrc/dp_ri_mt.c:133-205): Jaumann trial stress, then the
+ * rate-independent Drucker-Prager return tau <= mu_s p; szz is state[1]. */
+static void dp_ri_mt_point(particle_t *p, double dt, double G, double K, double mu_s)
+{
+    const double lambda = K - 2.0 * G / 3.0;
+    double *szz = &p->state[1];
+    const double trD = p->exx_t + p->eyy_t;
+    double dsjxx = lambda * trD + 2.0 * G * p->exx_t;
+    double dsjxy = 2.0 * G * p->exy_t;
+    double dsjyy = lambda * trD + 2.0 * G * p->eyy_t;
+    dsjxx += 2 * p->wxy_t * p->sxy;
+    dsjxy -= p->wxy_t * (p->sxx - p->syy);
+    dsjyy -= 2 * p->wxy_t * p->sxy;
+    const double sxx_tr = p->sxx + dt * dsjxx;
+    const double sxy_tr = p->sxy + dt * dsjxy;
+    const double syy_tr = p->syy + dt * dsjyy;
+    const double szz_tr = *szz + dt * lambda * trD;
+    const double p_tr = -(sxx_tr + syy_tr + szz_tr) / 3.0;
+    const double t0xx_tr = sxx_tr + p_tr;
+    const double t0xy_tr = sxy_tr;
+    const double t0yy_tr = syy_tr + p_tr;
+    const double t0zz_tr = szz_tr + p_tr;
+    const double tau_tr = sqrt(0.5 * (t0xx_tr * t0xx_tr + 2 * t0xy_tr * t0xy_tr
+                                      + t0yy_tr * t0yy_tr + t0zz_tr * t0zz_tr));
+    double nup_tau;
+    if ((p->m / p->v) < 1485.0f || p_tr <= 0) {
+        nup_tau = (tau_tr) / (G * dt);
+        p->sxx = 0;
+        p->sxy = 0;
+        p->syy = 0;
+        *szz = 0;
+    } else if (p_tr > 0) {
+        const double S0 = mu_s * p_tr;
+        double tau_tau, scale_factor;
+        if (tau_tr <= S0) {
+            tau_tau = tau_tr;
+            scale_factor = 1.0;
+        } else {
+            tau_tau = S0;
+            scale_factor = (tau_tau / tau_tr);
+        }
+        nup_tau = ((tau_tr - tau_tau) / G) / dt;
+        p->sxx = scale_factor * t0xx_tr - p_tr;
+        p->sxy = scale_factor * t0xy_tr;
+        p->syy = scale_factor * t0yy_tr - p_tr;
+        *szz = scale_factor * t0zz_tr - p_tr;
+    } else {
+        nup_tau = 0;
+    }
+    p->state[9] += nup_tau * dt;
+    p->state[10] = nup_tau;
+}
+
+static void dp_ri_mt_stress_threaded(threadtask_t *task)
+{
+    oracle_job_t *oj = OJ(task->job);
+    job_t *job = task->job;
+    for (size_t i = task->offset; i < task->offset + task->blocksize; i++) {
+        if (job->active[i] == 0) {
+            continue;
+        }
+        dp_ri_mt_point(&job->particles[i], job->dt, oj->G, oj->K, oj->prm.mu_s);
+    }
 }
 
 static void isolin_stress_threaded(threadtask_t *task)
@@ -1336,6 +1403,8 @@ static void stress_update(threadtask_t *task)
         }
     } else if (oj->prm.material_kind == 2) {
         isolin_stress_threaded(task);
+    } else if (oj->prm.material_kind == 3) {
+        dp_ri_mt_stress_threaded(task);
     } else {
         g_local_mu2_stress_threaded(task);
     }
@@ -1643,7 +1712,7 @@ enum {
     OF_X, OF_Y, OF_VX, OF_VY, OF_M, OF_V, OF_TXX, OF_TXY, OF_TYY, OF_TZZ,
     OF_GAMMAP, OF_GAMMADOTP, OF_UX, OF_UY, OF_EXX, OF_EXY, OF_EYY, OF_WXY,
     OF_AX, OF_AY, OF_XL, OF_YL, OF_SXX, OF_SXY, OF_SYY, OF_TXZ, OF_TYX,
-    OF_TYZ, OF_TZX, OF_TZY, OF_BY, OF_GF, OF_GF_BULK, OF_XISQ_INV, OF_NFIELDS
+    OF_TYZ, OF_TZX, OF_TZY, OF_BY, OF_GF, OF_GF_BULK, OF_XISQ_INV, OF_STATE1, OF_NFIELDS
 };
 
 int oracle_num_particle_fields(void)
@@ -1663,7 +1732,7 @@ void oracle_get_particles(void *handle, double *out, int *active, int *in_elemen
             p->T[ZZ], p->state[9], p->state[10], p->ux, p->uy, p->exx_t, p->exy_t,
             p->eyy_t, p->wxy_t, p->x_tt, p->y_tt, p->xl, p->yl, p->sxx, p->sxy,
             p->syy, p->T[XZ], p->T[YX], p->T[YZ], p->T[ZX], p->T[ZY], p->by,
-            p->state[3], p->state[5], p->state[10] };
+            p->state[3], p->state[5], p->state[10], p->state[1] };
         for (int k = 0; k < OF_NFIELDS; k++) {
             out[(size_t)k * np + i] = f[k];
         }
@@ -1776,6 +1845,35 @@ void oracle_isolin_point(const double *sin, double exx, double exy, double eyy,
     out[0] = p.sxx;
     out[1] = p.sxy;
     out[2] = p.syy;
+}
+
+/* One dp_ri_mt update: s = {sxx, sxy, syy, szz, gammap} -> out = {sxx,
+ * sxy, syy, szz, gammap, gammadotp} (test hook). */
+void oracle_dp_ri_mt_point(const double *sin, double m, double v, double exx, double exy,
+    double eyy, double wxy, double dt, const oracle_params_t *q, double *out)
+{
+    particle_t p;
+    memset(&p, 0, sizeof p);
+    p.sxx = sin[0];
+    p.sxy = sin[1];
+    p.syy = sin[2];
+    p.state[1] = sin[3];
+    p.state[9] = sin[4];
+    p.m = m;
+    p.v = v;
+    p.exx_t = exx;
+    p.exy_t = exy;
+    p.eyy_t = eyy;
+    p.wxy_t = wxy;
+    double G = q->E / (2.0 * (1.0 + q->nu));
+    double K = q->E / (3.0 * (1.0 - 2 * q->nu));
+    dp_ri_mt_point(&p, dt, G, K, q->mu_s);
+    out[0] = p.sxx;
+    out[1] = p.sxy;
+    out[2] = p.syy;
+    out[3] = p.state[1];
+    out[4] = p.state[9];
+    out[5] = p.state[10];
 }
 
 void oracle_stress_point(const double *Tin, double m, double v, double gammap,
