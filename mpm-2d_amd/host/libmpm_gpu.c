@@ -578,7 +578,10 @@ static void gpu_setup(job_t *job)
     }
     CHECK(mpm_hip_set_particles(G.ctx, m, v, x, y, vx, vy, sxx, sxy, syy));
     const struct { int f; size_t off; } extra[] = {
-        {MPM_F_TZZ, offsetof(particle_t, T) + ZZ * sizeof(double)},
+        /* the law's out-of-plane stress: T[ZZ] for g_local_mu2, szz =
+         * state[1] for dp_ri_mt (a restarted job carries both) */
+        {MPM_F_TZZ, G.s_law == 2 ? offsetof(particle_t, state) + 1 * sizeof(double)
+                                 : offsetof(particle_t, T) + ZZ * sizeof(double)},
         {MPM_F_GAMMAP, offsetof(particle_t, state) + 9 * sizeof(double)},
         {MPM_F_GAMMADOTP, offsetof(particle_t, state) + 10 * sizeof(double)},
         {MPM_F_UX, offsetof(particle_t, ux)},

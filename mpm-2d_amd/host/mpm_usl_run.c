@@ -4,9 +4,9 @@
  * to end without libconfuse (absent here; the reference driver needs it for
  * its .cfg parsing).  Two ways to call it:
  *
- *   mpm_usl_run -c SIM.cfg [-o DIR] [-p PFILE] [-g GFILE] [-u MAT.so]
- *               [-b BC.so] [-t THREADS] [--musl] [--state FILE] [--check-cfg]
- *               [t_max]
+ *   mpm_usl_run -c SIM.cfg [-o DIR] [-r STATE] [-p PFILE] [-g GFILE]
+ *               [-u MAT.so] [-b BC.so] [-t THREADS] [--musl] [--state FILE]
+ *               [--check-cfg] [t_max]
  *     the reference's command line (src/main.c:64, 145-160; t_max defaults
  *     to 1) and configuration file (host/cfg.h reads the libconfuse subset
  *     the reference's .cfg files use; options and defaults as
@@ -21,6 +21,11 @@
  *               [--musl] [--state FILE]
  *     the same loop with t_stop = (K - 1/2) dt, so it runs exactly K steps.
  *
+ * -r STATE restarts from a state file (write_state's format): grid,
+ * particles, clock, loads, state variables and active flags come from it
+ * (the plane-strain T[ZZ] of g_local_mu2 is not in the format and restarts
+ * at zero, as material_init sets it); with the flags form, --steps counts
+ * from the saved time.
  * --check-cfg prints the settings a -c run would use and exits.
  * --musl steps with explicit_mpm_step_musl_threaded (process_usl.c:284-462;
  * the reference's loop has that call commented out, src/main.c:948).
@@ -186,6 +191,71 @@ static void write_state(FILE *fd, const job_t *job)
     }
 }
 
+/* The state file written by write_state, read back for a restart (the
+ * format of src/reader.c:123-243; the reference's reader writes through an
+ * unallocated active array and returns a job without colours or plugins,
+ * src/reader.c:215 -- here the job is rebuilt by mpm_init and the saved
+ * per-particle fields are put back after the material's init). */
+typedef struct saved_state {
+    double t, dt, t_stop, h;
+    size_t np, nn, ne, N;
+    particle_t *p;
+    int *active;
+} saved_state;
+
+static int read_state(const char *path, saved_state *st)
+{
+    FILE *fd = fopen(path, "r");
+    if (!fd) {
+        fprintf(stderr, "cannot open state file '%s'\n", path);
+        return -1;
+    }
+    memset(st, 0, sizeof *st);
+    int r = fscanf(fd, "%lg %lg %lg", &st->t, &st->dt, &st->t_stop);
+    r += fscanf(fd, "%zu %zu %zu", &st->np, &st->nn, &st->ne);
+    r += fscanf(fd, "%zu %lg", &st->N, &st->h);
+    if (r != 8 || st->N < 2 || st->nn != st->N * st->N) {
+        fprintf(stderr, "Error reading state header!\n");
+        fclose(fd);
+        return -1;
+    }
+    st->p = calloc(st->np ? st->np : 1, sizeof(particle_t));
+    st->active = calloc(st->np ? st->np : 1, sizeof(int));
+    for (size_t i = 0; i < st->np; i++) {
+        particle_t *q = &st->p[i];
+        int dep = 0;
+        r = fscanf(fd, "%lg %lg", &q->x, &q->y);
+        r += fscanf(fd, "%lg %lg", &q->xl, &q->yl);
+        r += fscanf(fd, "%lg %lg", &q->x_t, &q->y_t);
+        r += fscanf(fd, "%lg", &q->m);
+        r += fscanf(fd, "%lg", &q->v);
+        r += fscanf(fd, "%lg", &q->v0);
+        r += fscanf(fd, "%lg %lg %lg", &q->sxx, &q->sxy, &q->syy);
+        r += fscanf(fd, "%lg %lg %lg %lg", &q->exx_t, &q->exy_t, &q->eyy_t, &q->wxy_t);
+        r += fscanf(fd, "%lg %lg", &q->bx, &q->by);
+        r += fscanf(fd, "%lg %lg %lg %lg", &q->Fxx, &q->Fxy, &q->Fyx, &q->Fyy);
+        r += fscanf(fd, "%lg %lg", &q->ux, &q->uy);
+        r += fscanf(fd, "%lg", &q->color);
+        r += fscanf(fd, "%d", &dep);
+        if (dep < 0 || dep > DEPVAR) {
+            fprintf(stderr, "error reading state of particle %zu\n", i);
+            fclose(fd);
+            return -1;
+        }
+        for (int j = 0; j < dep; j++) {
+            r += fscanf(fd, "%lg", &q->state[j]);
+        }
+        r += fscanf(fd, "%d", &st->active[i]);
+        if (r != 26 + dep + 1) {
+            fprintf(stderr, "error reading state of particle %zu\n", i);
+            fclose(fd);
+            return -1;
+        }
+    }
+    fclose(fd);   /* node coordinates and element nodes follow; mpm_init rebuilds them */
+    return 0;
+}
+
 static void write_hex_state(const char *path, const job_t *job)
 {
     FILE *fd = fopen(path, "w");
@@ -204,7 +274,7 @@ static void write_hex_state(const char *path, const job_t *job)
 }
 
 typedef struct run_opts {
-    const char *cfgfile, *outdir, *grid, *pfile, *mat, *bc, *frames, *state;
+    const char *cfgfile, *outdir, *grid, *pfile, *mat, *bc, *frames, *state, *restart;
     double dt, rate, t_stop;
     int steps, have_t_stop, check_cfg;
     size_t T;
@@ -306,6 +376,7 @@ int main(int argc, char **argv)
         const char *v = a + 1 < argc ? argv[a + 1] : "";
         if (!strcmp(k, "-c")) o.cfgfile = v, a++;
         else if (!strcmp(k, "-o")) o.outdir = v, a++;
+        else if (!strcmp(k, "-r")) o.restart = v, a++;
         else if (!strcmp(k, "-g")) o.grid = v, a++;
         else if (!strcmp(k, "-p")) o.pfile = v, a++;
         else if (!strcmp(k, "-u")) o.mat = v, a++;
@@ -344,7 +415,8 @@ int main(int argc, char **argv)
         }
         if (!o.pfile) o.pfile = cfg_str(&cf, "input|initial-particle-file", "particles.txt");
         if (!o.grid) o.grid = cfg_str(&cf, "input|grid-file", "grid.txt");
-    } else if (!o.grid || !o.pfile || !o.mat || !o.bc || o.dt <= 0 || o.steps <= 0) {
+    } else if ((!o.restart && (!o.grid || !o.pfile || o.dt <= 0)) || !o.mat || !o.bc
+               || o.steps <= 0) {
         fprintf(stderr, "usage: mpm_usl_run -c SIM.cfg [-o DIR] [-p PF] [-g GF] [-u MAT.so] "
                 "[-b BC.so] [-t T] [--musl] [t_max]\n"
                 "   or: mpm_usl_run -g GRID -p PARTICLES -u MAT.so -b BC.so --dt DT --steps K "
@@ -374,19 +446,37 @@ int main(int argc, char **argv)
         return 0;
     }
     int N;
-    double len;
+    double h;
     particle_t *pdata;
     size_t np;
-    if (read_grid(o.grid, &N, &len)) {
-        fprintf(stderr, "Error reading grid file.\n");
-        return EXIT_ERROR_CFG_PARSE;
+    saved_state saved;
+    memset(&saved, 0, sizeof saved);
+    if (o.restart) {
+        /* -r STATE (src/main.c:326-332): grid and particles from the state */
+        if (read_state(o.restart, &saved)) {
+            return EXIT_ERROR_CFG_PARSE;
+        }
+        N = (int)saved.N;
+        h = saved.h;
+        np = saved.np;
+        pdata = calloc(np ? np : 1, sizeof(particle_t));
+        memcpy(pdata, saved.p, np * sizeof(particle_t));
+        if (!cfg_mode) o.dt = saved.dt;
+    } else {
+        double len;
+        if (read_grid(o.grid, &N, &len)) {
+            fprintf(stderr, "Error reading grid file.\n");
+            return EXIT_ERROR_CFG_PARSE;
+        }
+        if (read_particles(o.pfile, &pdata, &np)) {
+            fprintf(stderr, "Error reading particle file.\n");
+            return EXIT_ERROR_CFG_PARSE;
+        }
+        h = len / (N - 1);
     }
-    if (read_particles(o.pfile, &pdata, &np)) {
-        fprintf(stderr, "Error reading particle file.\n");
-        return EXIT_ERROR_CFG_PARSE;
-    }
-    double h = len / (N - 1);
-    const double t_stop = cfg_mode ? (o.have_t_stop ? o.t_stop : 1.0) : (o.steps - 0.5) * o.dt;
+    const double t_begin = o.restart ? saved.t : 0.0;
+    const double t_stop = cfg_mode ? (o.have_t_stop ? o.t_stop : (o.restart ? saved.t_stop : 1.0))
+                                   : t_begin + (o.steps - 0.5) * o.dt;
     job_t *job = mpm_init(N, h, pdata, np, t_stop);
 
     char *dir = NULL;
@@ -398,6 +488,7 @@ int main(int argc, char **argv)
          * automatic-dt = 0 */
         const double cfg_dt = cfg_float(&cf, "timestep|dt", 1e-6);
         job->dt = cfg_int(&cf, "timestep|automatic-dt", 1) != 0 ? job->dt : cfg_dt;
+        if (o.restart) job->dt = saved.dt;
         /* output files (src/main.c:660-715) */
         const char *d = o.outdir ? o.outdir : cfg_str(&cf, "output|directory", "jobs/");
         struct stat st;
@@ -477,6 +568,33 @@ int main(int argc, char **argv)
     find_filled_elements(job);
     initial_loads(job);
     job->material.material_init(job);
+    if (o.restart) {
+        /* continue the saved run: its clock, loads, and the per-particle
+         * fields mpm_init and material_init reset */
+        job->t = saved.t;
+        for (size_t i = 0; i < np; i++) {
+            particle_t *q = &job->particles[i];
+            const particle_t *sp = &saved.p[i];
+            q->xl = sp->xl;
+            q->yl = sp->yl;
+            q->v0 = sp->v0;
+            q->exx_t = sp->exx_t;
+            q->exy_t = sp->exy_t;
+            q->eyy_t = sp->eyy_t;
+            q->wxy_t = sp->wxy_t;
+            q->bx = sp->bx;
+            q->by = sp->by;
+            q->Fxx = sp->Fxx;
+            q->Fxy = sp->Fxy;
+            q->Fyx = sp->Fyx;
+            q->Fyy = sp->Fyy;
+            q->ux = sp->ux;
+            q->uy = sp->uy;
+            q->color = sp->color;
+            memcpy(q->state, sp->state, sizeof q->state);
+            job->active[i] = saved.active[i];
+        }
+    }
     if (job->boundary.bc_validate(job) == 0) {
         fprintf(stderr, "Error with boundary condition properties.\n");
         return EXIT_ERROR_BC_PROPS;

@@ -70,16 +70,15 @@ output
 """
 
 
-def write_cfg(tmp_path, w, rate=20000.0, plugin_dir="lib"):
+def write_cfg(tmp_path, w, rate=20000.0, plugin_dir="lib", mat="g_local_mu2",
+              keys=("E", "nu", "mu_s", "mu_2", "I_0", "grains_d", "grains_rho", "rho_crit")):
     g, p = tmp_path / "grid.txt", tmp_path / "particles.txt"
     workloads.write_reference_inputs(w, g, p)
     out = tmp_path / "out"
     out.mkdir(exist_ok=True)
     m = w["material"]
-    text = CFG.format(dt=w["dt"], mat=os.path.join(PKG, plugin_dir, "g_local_mu2.so"),
-                      mprops=", ".join(repr(float(m[k])) for k in
-                                       ("E", "nu", "mu_s", "mu_2", "I_0", "grains_d",
-                                        "grains_rho", "rho_crit")),
+    text = CFG.format(dt=w["dt"], mat=os.path.join(PKG, plugin_dir, mat + ".so"),
+                      mprops=", ".join(repr(float(m[k])) for k in keys),
                       bc=os.path.join(PKG, plugin_dir, "box.so"),
                       bprops=", ".join(repr(float(v)) for v in w["bc_fp64"]),
                       biprops=", ".join(str(int(v)) for v in w["bc_int"]),
@@ -262,3 +261,66 @@ def test_cfg_run_frames_and_state(tmp_path):
         assert blk[13 + 9] == "%g" % st["gammap"][i]
         assert blk[13 + 10] == "%g" % st["gammadotp"][i]
         assert blk[24] == str(int(st["active"][i]))
+
+
+def read_ref_state(path):
+    """write_state's format (src/writer.c:266-380): header and per-particle
+    blocks of 25 lines."""
+    lines = open(path).read().split("\n")
+    t, dt, t_stop = (float(v) for v in lines[0].split())
+    n = int(lines[1].split()[0])
+    per = 25
+    out = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "sxx", "sxy", "syy", "gammap")}
+    out["active"] = np.zeros(n, dtype=np.int32)
+    for i in range(n):
+        b = lines[3 + i * per: 3 + (i + 1) * per]
+        out["x"][i], out["y"][i] = (float(v) for v in b[0].split())
+        out["vx"][i], out["vy"][i] = (float(v) for v in b[2].split())
+        out["sxx"][i], out["sxy"][i], out["syy"][i] = (float(v) for v in b[6].split())
+        out["gammap"][i] = float(b[13 + 9])
+        out["active"][i] = int(b[24])
+    return t, dt, t_stop, out
+
+
+@pytest.mark.gpu
+def test_cfg_restart_continues_the_run(tmp_path):
+    """-r STATE continues a run from write_state's file: the clock, loads,
+    stress, state variables (dp_ri_mt keeps szz and the plastic strain
+    there) and active flags carry over, and the result stays within the
+    6-significant-digit rounding of the state format of an uninterrupted
+    run."""
+    w = workloads.small_box(cells=16, bx=5, by=9,
+                            material=dict(workloads.REFERENCE_MATERIAL, mu_s=0.4))
+    w["material_kind"] = 3
+    w["particles"] = workloads.lithostatic(w["particles"], lambda x: 9.0 / 16 + 0 * x,
+                                           rho=1500.0, g=9.8, k0=0.4)
+    dt = w["dt"]
+    path, outdir = write_cfg(tmp_path, w, rate=100.0, mat="dp_ri_mt", keys=("E", "nu", "mu_s"))
+    print("case restart: first leg", flush=True)
+    r = subprocess.run([RUN, "-c", str(path), "-t", "2", repr(30.5 * dt)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    t1, _, _, s1 = read_ref_state(outdir / "state.txt")
+    assert abs(t1 - 31 * dt) < 1e-9
+    leg2 = tmp_path / "leg2"
+    leg2.mkdir()
+    (tmp_path / "first_state.txt").write_text((outdir / "state.txt").read_text())
+    print("case restart: second leg", flush=True)
+    r = subprocess.run([RUN, "-c", str(path), "-o", str(leg2), "-r",
+                        str(tmp_path / "first_state.txt"), "-t", "2", repr(60.5 * dt)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    t2, dt2, _, s2 = read_ref_state(leg2 / "state.txt")
+    assert dt2 == dt and abs(t2 - 61 * dt) < 1e-9
+    # the uninterrupted run on the oracle
+    o = pyoracle.from_workload(w, num_threads=2)
+    o.run(61)
+    ref = o.particles()
+    assert abs(o.time() - t2) < 1e-9
+    for k, rk in (("x", "x"), ("y", "y"), ("vx", "vx"), ("vy", "vy"), ("sxx", "sxx"),
+                  ("syy", "syy")):
+        scale = np.abs(ref[rk]).max()
+        err = np.abs(s2[k] - ref[rk]).max()
+        assert err <= 1e-4 * scale, f"{k}: {err} vs scale {scale}"
+    assert np.array_equal(s2["active"], ref["active"])
+    assert s1["gammap"].max() > 0 and np.all(s2["gammap"] >= s1["gammap"] - 1e-12)
