@@ -974,7 +974,8 @@ struct NlSystem {
 };
 
 constexpr int kNlThreads = 256;
-constexpr int kCgThreads = 1024;
+constexpr int kCgTile = 32;                     // CG tiles: 32 x 32 nodes
+constexpr int kCgThreads = kCgTile * kCgTile;   // one thread per tile node
 constexpr int kNlMaxBlocks = 2048;
 
 // Assembly of K g = f (solve_diffusion_part, g_nonlocal.c:356-592) at the
@@ -1280,6 +1281,136 @@ __global__ __launch_bounds__(kCgThreads) void k_nl_cg(NlSystem S, double rel_tol
     const double stop = rel_tol * rel_tol * ff;
     double alpha = gamma / delta, beta = 0.0;
     int it = 0;
+    // Iterations work on 32 x 32-node tiles: each workgroup forms the new u
+    // of its tile and of the one-node halo around it in LDS, then takes the
+    // stencil product from there.
+    constexpr int E = kCgTile + 2;
+    __shared__ double s_u[E * E];
+    const int tiles_x = (N + kCgTile - 1) / kCgTile;
+    const int ntiles = tiles_x * tiles_x;
+    const int lx = threadIdx.x % kCgTile, ly = threadIdx.x / kCgTile;
+    int hx = -2, hy = -2;   // this thread's halo node, if any
+    {
+        const int t = threadIdx.x;
+        if (t < E) {
+            hx = t - 1;
+            hy = -1;
+        } else if (t < 2 * E) {
+            hx = t - E - 1;
+            hy = kCgTile;
+        } else if (t < 2 * E + kCgTile) {
+            hx = -1;
+            hy = t - 2 * E;
+        } else if (t < 2 * E + 2 * kCgTile) {
+            hx = kCgTile;
+            hy = t - 2 * E - kCgTile;
+        }
+    }
+    if (ntiles <= nb) {
+        // One tile per workgroup for the whole solve: each thread keeps its
+        // node's vectors and stencil row in registers; only the halo comes
+        // from memory, loaded right after each grid barrier together with
+        // the partial sums, so an iteration waits on memory once.
+        const bool have_tile = (int)blockIdx.x < ntiles;
+        const int x0 = have_tile ? ((int)blockIdx.x % tiles_x) * kCgTile : 0;
+        const int y0 = have_tile ? ((int)blockIdx.x / tiles_x) * kCgTile : 0;
+        const int ix = x0 + lx, iy = y0 + ly;
+        const bool own = have_tile && ix < N && iy < N;
+        const int i = own ? iy * N + ix : 0;
+        double Kr[9];
+        double u = 0.0, w = 0.0, sv = 0.0, pv = 0.0, xv = 0.0, rv = 0.0, dv = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Kr[k] = own ? K[(size_t)k * nn + i] : 0.0;
+        }
+        if (own) {
+            u = S.u[i];
+            w = S.w[i];
+            xv = S.x[i];
+            rv = S.r[i];
+            dv = dinv[i];
+        }
+        // halo values of the iteration about to run (buffer 0 first)
+        const int hxg = x0 + hx, hyg = y0 + hy;
+        const bool hin = have_tile && hx != -2 && hxg >= 0 && hxg < N && hyg >= 0 && hyg < N;
+        const int hj = hin ? hyg * N + hxg : 0;
+        double hu = 0.0, hw = 0.0, hs = 0.0, hd = 0.0;
+        if (hin) {
+            hu = S.u[hj];
+            hw = S.w[hj];
+            hs = S.s[hj];
+            hd = dinv[hj];
+        }
+        while (it < max_iter && rr > stop) {
+            const int nxt = (it + 1) & 1;
+            double gl = 0.0, dll = 0.0, rrl = 0.0;
+            const double si = w + beta * sv;
+            const double ui = u - alpha * (dv * si);
+            double uh = 0.0;
+            if (hin) {
+                const double sj = hw + beta * hs;
+                uh = hu - alpha * (hd * sj);
+            }
+            s_u[(ly + 1) * E + lx + 1] = own ? ui : 0.0;
+            if (hx != -2) {
+                s_u[(hy + 1) * E + hx + 1] = uh;
+            }
+            __syncthreads();
+            if (own) {
+                double wv = 0.0;
+#pragma unroll
+                for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int jx = ix + ox, jy = iy + oy;
+                        if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
+                            wv += Kr[(oy + 1) * 3 + (ox + 1)]
+                                  * s_u[(ly + 1 + oy) * E + lx + 1 + ox];
+                        }
+                    }
+                }
+                pv = u + beta * pv;
+                xv += alpha * pv;
+                rv = rv - alpha * si;
+                sv = si;
+                u = ui;
+                w = wv;
+                S.u[(size_t)nxt * nn + i] = u;
+                S.w[(size_t)nxt * nn + i] = w;
+                S.s[(size_t)nxt * nn + i] = sv;
+                gl += rv * u;
+                dll += w * u;
+                rrl += rv * rv;
+            }
+            double *pt = part + (size_t)nxt * 4 * kNlMaxBlocks;
+            block_sum3(gl, dll, rrl, s_red);   // (its barriers also retire s_u's readers)
+            if (threadIdx.x == 0) {
+                pt[blockIdx.x] = gl;
+                pt[kNlMaxBlocks + blockIdx.x] = dll;
+                pt[2 * kNlMaxBlocks + blockIdx.x] = rrl;
+            }
+            grid_barrier(S.barrier, target);
+            if (hin) {
+                hu = S.u[(size_t)nxt * nn + hj];
+                hw = S.w[(size_t)nxt * nn + hj];
+                hs = S.s[(size_t)nxt * nn + hj];
+            }
+            double gamma_new, delta_new;
+            grid_total3(pt, nb, gamma_new, delta_new, rr, s_red);
+            beta = gamma_new / gamma;
+            alpha = gamma_new / (delta_new - beta * gamma_new / alpha);
+            gamma = gamma_new;
+            ++it;
+        }
+        if (own) {
+            S.x[i] = xv;
+        }
+        if (gid == 0) {
+            S.stats[0] = it;
+            S.res[0] = ff > 0.0 ? sqrt(rr / ff) : 0.0;
+        }
+        return;
+    }
     while (it < max_iter && rr > stop) {
         const int cur = it & 1;
         const double *uo = S.u + (size_t)cur * nn, *wo = S.w + (size_t)cur * nn;
@@ -1287,35 +1418,56 @@ __global__ __launch_bounds__(kCgThreads) void k_nl_cg(NlSystem S, double rel_tol
         double *un = S.u + (size_t)(cur ^ 1) * nn, *wn = S.w + (size_t)(cur ^ 1) * nn;
         double *sn = S.s + (size_t)(cur ^ 1) * nn;
         double gl = 0.0, dll = 0.0, rrl = 0.0;
-        for (int i = gid; i < nn; i += stride) {
-            const int ix = i % N, iy = i / N;
-            double w = 0.0;
-#pragma unroll
-            for (int oy = -1; oy <= 1; ++oy) {
-#pragma unroll
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const int jx = ix + ox, jy = iy + oy;
-                    if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
-                        const int j = jy * N + jx;
-                        const double sj = wo[j] + beta * so[j];
-                        const double uj = uo[j] - alpha * (dinv[j] * sj);
-                        w += K[(size_t)((oy + 1) * 3 + (ox + 1)) * nn + i] * uj;
-                    }
+        for (int tile = blockIdx.x; tile < ntiles; tile += nb) {
+            const int x0 = (tile % tiles_x) * kCgTile, y0 = (tile / tiles_x) * kCgTile;
+            const int ix = x0 + lx, iy = y0 + ly;
+            const bool own = ix < N && iy < N;
+            const int i = iy * N + ix;
+            double si = 0.0, ui = 0.0;
+            if (own) {
+                si = wo[i] + beta * so[i];
+                ui = uo[i] - alpha * (dinv[i] * si);
+            }
+            double uh = 0.0;
+            if (hx != -2) {
+                const int jx = x0 + hx, jy = y0 + hy;
+                if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
+                    const int j = jy * N + jx;
+                    const double sj = wo[j] + beta * so[j];
+                    uh = uo[j] - alpha * (dinv[j] * sj);
                 }
             }
-            const double si = wo[i] + beta * so[i];
-            const double ui = uo[i] - alpha * (dinv[i] * si);
-            const double pi = uo[i] + beta * S.p[i];
-            S.p[i] = pi;
-            S.x[i] += alpha * pi;
-            const double ri = S.r[i] - alpha * si;
-            S.r[i] = ri;
-            sn[i] = si;
-            un[i] = ui;
-            wn[i] = w;
-            gl += ri * ui;
-            dll += w * ui;
-            rrl += ri * ri;
+            __syncthreads();   // the previous tile's products are done with s_u
+            s_u[(ly + 1) * E + lx + 1] = ui;
+            if (hx != -2) {
+                s_u[(hy + 1) * E + hx + 1] = uh;
+            }
+            __syncthreads();
+            if (own) {
+                double w = 0.0;
+#pragma unroll
+                for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        const int jx = ix + ox, jy = iy + oy;
+                        if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
+                            w += K[(size_t)((oy + 1) * 3 + (ox + 1)) * nn + i]
+                                 * s_u[(ly + 1 + oy) * E + lx + 1 + ox];
+                        }
+                    }
+                }
+                const double pi = uo[i] + beta * S.p[i];
+                S.p[i] = pi;
+                S.x[i] += alpha * pi;
+                const double ri = S.r[i] - alpha * si;
+                S.r[i] = ri;
+                sn[i] = si;
+                un[i] = ui;
+                wn[i] = w;
+                gl += ri * ui;
+                dll += w * ui;
+                rrl += ri * ri;
+            }
         }
         double *pt = part + (size_t)((it + 1) & 1) * 4 * kNlMaxBlocks;
         block_sum3(gl, dll, rrl, s_red);
@@ -1814,6 +1966,39 @@ __global__ __launch_bounds__(kG2PThreads) void k_adj_p2g(
     A.lam[MPM_F_TXX][p] += -v * txxb;
     A.lam[MPM_F_TXY][p] += -v * txyb;
     A.lam[MPM_F_TYY][p] += -v * tyyb;
+}
+
+// Copies a particle state (the 14 step fields, active flags, device
+// scalars) in one launch, and optionally clears the two bin-count arrays
+// for the re-binning that follows a load.
+struct StateCopy {
+    const double *src[MPM_F_EXX];
+    double *dst[MPM_F_EXX];
+    const int *asrc;
+    int *adst;
+    const DevState *ssrc;
+    DevState *sdst;
+    int *cnt0, *cnt1;
+    int np, ncells;
+};
+
+__global__ __launch_bounds__(256) void k_copy_state(StateCopy c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.np) {
+#pragma unroll
+        for (int f = 0; f < MPM_F_EXX; ++f) {
+            c.dst[f][i] = c.src[f][i];
+        }
+        c.adst[i] = c.asrc[i];
+    }
+    if (c.cnt0 && i < c.ncells) {
+        c.cnt0[i] = 0;
+        c.cnt1[i] = 0;
+    }
+    if (i == 0) {
+        *c.sdst = *c.ssrc;
+    }
 }
 
 // acc[k] += sum over workgroups of the step's parameter partials, in order.
@@ -2417,7 +2602,8 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
             return MPM_HIP_ERR_DEVICE;
         }
         // co-resident workgroups only (the solve synchronises the grid)
-        long want = ((long)nn + kCgThreads - 1) / kCgThreads;
+        const long tx = (cfg->N + kCgTile - 1) / kCgTile;
+        long want = tx * tx;   // one workgroup per tile when they fit
         long have = (long)occ * prop.multiProcessorCount;
         long nbk = want < have ? want : have;
         ctx->cg_blocks = (int)(nbk < kNlMaxBlocks ? nbk : kNlMaxBlocks);
@@ -3273,25 +3459,39 @@ struct mpm_hip_adjoint {
     bool have_particles, have_forward;
 };
 
+// Saves the context's particle state to (f, act, st) or loads it back; a
+// load also re-bins the particles (bins cleared in the same launch).
 static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, DevState *st)
 {
     mpm_hip_ctx *ctx = a->ctx;
-    const size_t bytes = a->np * sizeof(double);
+    StateCopy c;
+    memset(&c, 0, sizeof c);
     for (int k = 0; k < MPM_F_EXX; ++k) {
-        double *dev = ctx->pfield[k], *buf = f + (size_t)k * a->np;
-        if (bytes) {
-            HIP_TRY(hipMemcpyAsync(save ? buf : dev, save ? dev : buf, bytes,
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-        }
+        double *dev = ctx->pfield[k], *buf = f + (size_t)k * (a->np ? a->np : 1);
+        c.src[k] = save ? dev : buf;
+        c.dst[k] = save ? buf : dev;
     }
-    if (a->np) {
-        HIP_TRY(hipMemcpyAsync(save ? act : ctx->P.active, save ? ctx->P.active : act,
-                               a->np * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    HIP_TRY(hipMemcpyAsync(save ? st : ctx->st, save ? ctx->st : st, sizeof(DevState),
-                           hipMemcpyDeviceToDevice, ctx->stream));
+    c.asrc = save ? ctx->P.active : act;
+    c.adst = save ? act : ctx->P.active;
+    c.ssrc = save ? ctx->st : st;
+    c.sdst = save ? st : ctx->st;
+    c.np = (int)a->np;
     if (!save) {
-        return rebin(ctx, 0);
+        c.cnt0 = ctx->cnt[0];
+        c.cnt1 = ctx->cnt[1];
+        c.ncells = ctx->g.ncells;
+    }
+    const int n = c.np > c.ncells ? c.np : c.ncells;
+    k_copy_state<<<dim3((unsigned)((n > 0 ? n : 1) + 255) / 256), dim3(256), 0, ctx->stream>>>(c);
+    HIP_TRY(hipGetLastError());
+    if (!save) {
+        ctx->cur = 0;
+        if (a->np) {
+            k_bin_particles<false><<<dim3((unsigned)a->g2p_blocks), dim3(kG2PThreads), 0,
+                                     ctx->stream>>>(ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g,
+                                                    ctx->st, (int)a->np, 0, BatchTables{});
+            HIP_TRY(hipGetLastError());
+        }
     }
     return MPM_HIP_OK;
 }
