@@ -47,9 +47,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(REPO, "profiles", "hbm_traffic_silo_512.txt")
+def read_traffic(kernel, summary="hbm_traffic_silo_512.txt"):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(REPO, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -213,7 +213,8 @@ def bench_ensemble(args):
                    "members": len(ens), "particles": np_sum, "nodes": nn_sum,
                    "bin_capacity": cap, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(dom, "hbm_traffic_ensemble_256.txt"),
                      "kernel": dom, "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
         "kernels_ms": {k: v[0] for k, v in kernels.items()},
         "step_roofline": {"algorithmic_bytes": step_bytes,
@@ -347,8 +348,29 @@ def cpu_baseline_members(sample, budget_s):
                        f"x86-64-{variant} build, {threads} pthreads")}
 
 
+def copy_probe():
+    """A 256 MiB device-to-device copy through the HIP runtime, five times:
+    the known-size transfer that calibrates rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters (bytes per KiB reported) for `roofline.traffic`."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 256 * 1024 * 1024
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        sys.exit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(5):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3):   # hipMemcpyDeviceToDevice
+            sys.exit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the 256 MiB copy that calibrates the HBM traffic counters")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
@@ -362,6 +384,9 @@ def main():
                          "ensemble: 256 uneven members stepped as one batch; "
                          "adjoint: forward + reverse sweep of a 200-step run")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the USL step is built and timed on a single "
               "MI355X; multi-GPU sharding is a design note, not a measured path.",
