@@ -58,6 +58,13 @@ constexpr int kStageMax = 704;     // staged records per batch (64 B each, incl.
 constexpr int kTileNodes = kTileX * kTileY;
 constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
 constexpr int kG2PThreads = 256;
+constexpr int kBatchTileChunk = 16;   // ensemble P2G tiles per XCD chunk
+// Particle count limit: byte offsets into a particle field fit the 31-bit
+// range of a raw buffer load.
+constexpr size_t kMaxParticles = (size_t)1 << 28;
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle): loads past num_records return 0.
+constexpr int kBufferDword3 = 0x00020000;
 
 enum : int { kStatusBinOverflow = 1 };
 enum : int { kModeStoreAux = 1, kModeHostMaterial = 2, kModeNonlocal = 4, kModeIsolin = 8,
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
-    int tiles_x, int ntiles, BatchTables bt)
+    int tiles_x, int ntiles, int chunk, int np, BatchTables bt)
 {
     // bin entries of one colour's cells held per thread between colours
     constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
@@ -420,6 +427,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     __shared__ int s_cnt[kRegCells];
     __shared__ int s_lst[kColourCellsMax * kLstStride];
     __shared__ int s_off[kColourCellsMax + 1];
+    __shared__ int s_bq[kColourCellsMax + 1];   // staging batches: cells [s_bq[b], s_bq[b+1])
+    __shared__ int s_nbat;
+    __shared__ int s_wbits[kP2GThreads / 64];
     __shared__ unsigned char s_owner[kColourCellsMax * kLstStride];   // staged slot -> cell
     // staged particles of one colour, paired for 16-byte LDS reads
     __shared__ double2 s_xy[kStageMax];     // local coordinates xl, yl
@@ -427,10 +437,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
     __shared__ double2 s_pys[kStageMax];    // vy * m, -syy * v
 
-    // Consecutive tiles go to the same XCD (workgroups are dealt to the 8
-    // XCDs round-robin), so neighbouring tiles share that XCD's L2.
-    const int per_xcd = gridDim.x >> 3;
-    int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    // Workgroups are dealt to the 8 XCDs round-robin.  Chunks of `chunk`
+    // consecutive tiles (a tile row of a single grid) go to the XCDs in turn,
+    // so tiles of one chunk share an XCD's L2 while any patch of the grid
+    // that holds particles is spread over all eight XCDs.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    int tile = ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
     if (tile >= ntiles) {
         return;
     }
@@ -463,6 +475,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             cnt_next[rr * g.Ne + cc] = 0;
         }
     }
+    // Bin counts of the 17 x 9 cells around the tile, and the colours that
+    // hold any particle.  An empty colour is skipped: folding no particles
+    // leaves every node sum as it was.
+    int cbits = 0;
     for (int cl = tid; cl < kRegCells; cl += kP2GThreads) {
         const int cc = cx0 + cl % kRegX, rr = cy0 + cl / kRegX;
         int n = 0;
@@ -471,15 +487,33 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             n = n < CAP ? n : CAP;
         }
         s_cnt[cl] = n;
+        if (n > 0) {
+            cbits |= 1 << (2 * (rr & 1) + (cc & 1));
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        cbits |= __shfl_xor(cbits, d);
+    }
+    if ((tid & 63) == 0) {
+        s_wbits[tid >> 6] = cbits;
     }
     __syncthreads();
+    int cmask = 0;
+#pragma unroll
+    for (int w = 0; w < kP2GThreads / 64; ++w) {
+        cmask |= s_wbits[w];
+    }
+    cmask = __builtin_amdgcn_readfirstlane(cmask);
 
     // Colour c's cells: region column a = 2*ax + ca holds cell column cx0 + a
     // (cx0 is odd, so colour parity pc <-> ca = 1 - pc); likewise rows.
     auto ncx_of = [](int c) { return (c & 1) ? (kRegX + 1) / 2 : kRegX / 2; };
     auto ncy_of = [](int c) { return (c >> 1) ? (kRegY + 1) / 2 : kRegY / 2; };
 
-    // Issues the loads of colour c's bin entries (all slots at once).
+    // Issues the loads of colour c's bin entries (all slots at once).  Slots
+    // past a bin's count read a slot of an in-grid cell and are discarded, so
+    // no branch separates the loads from the later ones the colour issues.
     int lst_reg[kListSlots];
     auto fetch_lists = [&](int c) {
         const int ncx = ncx_of(c), nq = ncx * ncy_of(c);
@@ -488,17 +522,28 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         for (int u = 0; u < kListSlots; ++u) {
             const int idx = tid + u * kP2GThreads;
             const int q = idx / CAP, k = idx % CAP;
-            int v = -1;
-            if (q < nq) {
-                const int a = 2 * (q % ncx) + ca, b = 2 * (q / ncx) + cb;
-                if (k < s_cnt[b * kRegX + a]) {
-                    v = bkt_cur[(size_t)((cy0 + b) * g.Ne + (cx0 + a)) * CAP + k];
-                }
-            }
-            lst_reg[u] = v;
+            const int qq = q < nq ? q : nq - 1;
+            const int a = 2 * (qq % ncx) + ca, b = 2 * (qq / ncx) + cb;
+            const bool valid = q < nq && k < s_cnt[b * kRegX + a];
+            const int cc = min(max(cx0 + a, 0), g.Ne - 1);
+            const int rr = min(max(cy0 + b, 0), g.Ne - 1);
+            const int v = bkt_cur[(size_t)(rr * g.Ne + cc) * CAP + k];
+            lst_reg[u] = valid ? v : -1;
         }
     };
-    fetch_lists(0);
+
+    // Particle fields as raw buffers: a load past the end returns 0 without
+    // a memory request, which idle staging slots use instead of a branch.
+    const int pbytes = np * (int)sizeof(double);
+    auto rsrc = [&](double *f) {
+        return __builtin_amdgcn_make_buffer_rsrc(f, (short)0, pbytes, kBufferDword3);
+    };
+    const __amdgpu_buffer_rsrc_t r_x = rsrc(P.x), r_y = rsrc(P.y), r_m = rsrc(P.m);
+    const __amdgpu_buffer_rsrc_t r_v = rsrc(P.v), r_vx = rsrc(P.vx), r_vy = rsrc(P.vy);
+    const __amdgpu_buffer_rsrc_t r_sx = rsrc(P.txx), r_sy = rsrc(P.txy), r_sz = rsrc(P.tyy);
+    auto ld = [](__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    };
 
     const int i = tx0 + (nt % kTileX);
     const int j = ty0 + (nt / kTileX);
@@ -506,10 +551,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const double bb = role == 0 ? st->bx : st->by;   // body force of this role's chain
     double am = 0.0, ap = 0.0, af = 0.0;             // role 0: m, mx, fx; role 1: -, my, fy
 
+    if (cmask != 0) {
+    fetch_lists(0);
     // Colours in the reference's order; colour = 2 * (row % 2) + (col % 2).
     // Unrolled, so each colour's cell geometry is a compile-time constant.
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
+        if (!((cmask >> c) & 1)) {
+            if (c < 3) {
+                fetch_lists(c + 1);
+            }
+            continue;
+        }
         const int pc = c & 1, pr = c >> 1;
         const int ncx = ncx_of(c), ncy = ncy_of(c);
         const int ca = pc ? 0 : 1, cb = pr ? 0 : 1;
@@ -525,8 +578,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         if (tid < 64) {
             // staging offsets: an exclusive scan of (count + 1) over the
             // colour's cells in the first wave (the extra slot per cell
-            // staggers the cells' staged records across LDS banks), and the
-            // owner cell of every staged slot
+            // staggers the cells' staged records across LDS banks), the
+            // owner cell of every staged slot, and the batches of whole
+            // cells that fit the staging arrays
             int n = 0;
             if (tid < nq) {
                 n = s_cnt[(2 * (tid / ncx) + cb) * kRegX + 2 * (tid % ncx) + ca] + 1;
@@ -543,8 +597,24 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                     s_owner[t] = (unsigned char)tid;
                 }
             }
+            // incl rises by at least 1 per cell and a cell needs at most
+            // CAP + 1 <= kStageMax slots, so each batch is a nonempty run.
+            int q0 = 0, base = 0, nbat = 0;
+            while (q0 < nq) {
+                const unsigned long long fit =
+                    __ballot(tid >= q0 && tid < nq && incl - base <= kStageMax);
+                const int q1 = q0 + __popcll(fit);
+                ++nbat;
+                if (tid == 0) {
+                    s_bq[nbat] = q1;
+                }
+                base = __shfl(incl, q1 - 1);
+                q0 = q1;
+            }
             if (tid == 0) {
                 s_off[0] = 0;
+                s_bq[0] = 0;
+                s_nbat = nbat;
             }
         }
         if (tid >= kP2GThreads - kColourCellsMax && tid - (kP2GThreads - kColourCellsMax) < nq) {
@@ -565,9 +635,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
         }
         __syncthreads();
-        if (c < 3) {
-            fetch_lists(c + 1);   // in flight while this colour is staged and folded
-        }
 
         // This node's cell of colour c, as an index into the colour's cells.
         int myq = -1, dx = 0, dy = 0;
@@ -581,29 +648,25 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
         }
 
-        // Stage the colour's particles in batches of whole cells.  Cell q's
-        // records sit at [s_off[q], s_off[q+1] - 1); slot s_off[q+1] - 1 is
-        // padding.
-        int qb = 0;
-        while (qb < nq) {
+        // Stages the cells [qb, qe) of the colour (cell q's records at
+        // [s_off[q], s_off[q+1] - 1); slot s_off[q+1] - 1 is padding) and
+        // folds them into this node's chains.  The first batch also issues
+        // the next colour's bin loads, after its own particle loads.
+        auto batch = [&](int qb, int qe, bool first) {
             const int base = s_off[qb];
-            int qe = qb + 1;
-            while (qe < nq && s_off[qe + 1] - base <= kStageMax) {
-                ++qe;
-            }
             const int nb = s_off[qe] - base;
-            int sp[kStageSlots], sc[kStageSlots];
+            int off[kStageSlots], sc[kStageSlots];
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
                 const int s = tid + u * kP2GThreads;
-                sp[u] = 0;
+                off[u] = -(int)sizeof(double);   // past the end: idle slot
                 sc[u] = -1;
                 if (s < nb) {
                     const int gi = base + s;
                     const int lo = s_owner[gi];
                     const int k = gi - s_off[lo];
                     if (k < s_off[lo + 1] - s_off[lo] - 1) {   // not the padding slot
-                        sp[u] = s_lst[lo * kLstStride + k];
+                        off[u] = s_lst[lo * kLstStride + k] * (int)sizeof(double);
                         sc[u] = lo;
                     }
                 }
@@ -613,16 +676,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             double psx[kStageSlots], psy[kStageSlots], psz[kStageSlots];
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
-                const int p = sp[u];    // 0 for idle slots: always a valid index
-                px[u] = P.x[p];
-                py[u] = P.y[p];
-                pm[u] = P.m[p];
-                pv[u] = P.v[p];
-                pvx[u] = P.vx[p];
-                pvy[u] = P.vy[p];
-                psx[u] = P.txx[p];
-                psy[u] = P.txy[p];
-                psz[u] = P.tyy[p];
+                px[u] = ld(r_x, off[u]);
+                py[u] = ld(r_y, off[u]);
+                pm[u] = ld(r_m, off[u]);
+                pv[u] = ld(r_v, off[u]);
+                pvx[u] = ld(r_vx, off[u]);
+                pvy[u] = ld(r_vy, off[u]);
+                psx[u] = ld(r_sx, off[u]);
+                psy[u] = ld(r_sy, off[u]);
+                psz[u] = ld(r_sz, off[u]);
+            }
+            if (first && c < 3) {
+                fetch_lists(c + 1);   // in flight while this colour is staged and folded
             }
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
@@ -670,8 +735,13 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 }
             }
             __syncthreads();
-            qb = qe;
+        };
+        const int nbat = s_nbat;
+        batch(0, s_bq[1], true);
+        for (int bi = 1; bi < nbat; ++bi) {
+            batch(s_bq[bi], s_bq[bi + 1], false);
         }
+    }
     }
 
     // Hand the y chains to the role-0 thread of the node (staging space is
@@ -2115,6 +2185,14 @@ static int alloc_aux(mpm_hip_ctx *ctx)
     return MPM_HIP_OK;
 }
 
+// k_p2g_grid's grid: whole rounds of 8 chunks (one per XCD); workgroups
+// past the last tile return at once.
+static int p2g_grid_blocks(int ntiles, int chunk)
+{
+    const int chunks = (ntiles + chunk - 1) / chunk;
+    return ((chunks + 7) / 8) * 8 * chunk;
+}
+
 static int use_device(const mpm_hip_ctx *ctx)
 {
     HIP_TRY(hipSetDevice(ctx->device));
@@ -2130,17 +2208,17 @@ static void p2g_kernel_launch(mpm_hip_ctx *ctx)
     case 16:
         k_p2g_grid<16, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, BatchTables{});
+            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
     case 32:
         k_p2g_grid<32, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, BatchTables{});
+            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
     default:
         k_p2g_grid<64, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, BatchTables{});
+            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
     }
 }
@@ -2266,15 +2344,15 @@ static int batch_launch_p2g(mpm_hip_batch *b, int cur)
     switch (b->cap) {
     case 16:
         k_p2g_grid<16, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     case 32:
         k_p2g_grid<32, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     default:
         k_p2g_grid<64, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -2492,7 +2570,7 @@ const char *mpm_hip_status_string(int status)
 int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx **out)
 {
     if (!cfg || !out || cfg->N < 2 || !(cfg->h > 0.0) || !(cfg->dt > 0.0)
-        || num_particles > (size_t)0x7fffffff || cfg->N > 46340) {
+        || num_particles > kMaxParticles || cfg->N > 46340) {
         return MPM_HIP_ERR_INVALID;
     }
     int cap = cfg->bin_capacity ? cfg->bin_capacity : 32;
@@ -2537,7 +2615,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
 
     ctx->tiles_x = (cfg->N + kTileX - 1) / kTileX;
     ctx->ntiles = ctx->tiles_x * ((cfg->N + kTileY - 1) / kTileY);
-    ctx->p2g_blocks = ((ctx->ntiles + 7) / 8) * 8;
+    ctx->p2g_blocks = p2g_grid_blocks(ctx->ntiles, ctx->tiles_x);
 
 #define ALLOC_OR_FAIL(ptr, bytes)                                                   \
     do {                                                                            \
@@ -3021,7 +3099,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         nbase += (size_t)c->N * c->N;
         cbase += (size_t)mi.g.ncells;
         tbase += (long)mi.tiles_x * ((c->N + kTileY - 1) / kTileY);
-        if (pbase > (size_t)0x7fffffff || nbase > (size_t)0x7fffffff
+        if (pbase > kMaxParticles || nbase > (size_t)0x7fffffff
             || cbase * (size_t)64 > (size_t)0x7fffffff || tbase > 0x7fffffff / 8) {
             mpm_hip_batch_destroy(b);
             return MPM_HIP_ERR_INVALID;
@@ -3031,7 +3109,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     b->nn_total = nbase;
     b->ncells_total = cbase;
     b->ntiles = (int)tbase;
-    b->p2g_blocks = ((b->ntiles + 7) / 8) * 8;
+    b->p2g_blocks = p2g_grid_blocks(b->ntiles, kBatchTileChunk);
     b->g2p_blocks = (int)(pbase / kG2PThreads);
 
     if (hipSetDevice(device) != hipSuccess
