@@ -53,8 +53,8 @@ constexpr int kRegX = kTileX + 1;  // cells whose particles reach the tile: 17 x
 constexpr int kRegY = kTileY + 1;
 constexpr int kRegCells = kRegX * kRegY;
 constexpr int kColourCellsMax = ((kRegX + 1) / 2) * ((kRegY + 1) / 2);   // 9 x 5
-constexpr int kStageMax = 704;     // staged records per batch (64 B each, incl. one
-                                   // padding record per cell)
+constexpr int kStageMax = 720;     // staged records per batch (64 B each); a multiple
+                                   // of 16, the LDS bank swizzle's block
 constexpr int kTileNodes = kTileX * kTileY;
 constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
 constexpr int kG2PThreads = 256;
@@ -420,18 +420,21 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // bin entries of one colour's cells held per thread between colours
     constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
     constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;
-    // bins are stored CAP + 1 apart so the per-cell sort threads hit
-    // different LDS banks
-    constexpr int kLstStride = CAP + 1;
+    constexpr int kMaxBatches = 8;   // 45 cells of at most 64 particles: <= 5 batches
+    static_assert(kP2GThreads / 64 == 4, "one wave per colour scans the colour's cells");
 
     __shared__ int s_cnt[kRegCells];
-    __shared__ int s_lst[kColourCellsMax * kLstStride];
-    __shared__ int s_off[kColourCellsMax + 1];
-    __shared__ int s_bq[kColourCellsMax + 1];   // staging batches: cells [s_bq[b], s_bq[b+1])
-    __shared__ int s_nbat;
-    __shared__ int s_wbits[kP2GThreads / 64];
-    __shared__ unsigned char s_owner[kColourCellsMax * kLstStride];   // staged slot -> cell
-    // staged particles of one colour, paired for 16-byte LDS reads
+    // per colour: exclusive scan of the cells' counts (compact list offsets),
+    // staging batches of whole cells [s_bq[c][b], s_bq[c][b+1]) and their number
+    __shared__ int s_off[4][kColourCellsMax + 1];
+    __shared__ int s_bq[4][kMaxBatches + 1];
+    __shared__ int s_nbat[4];
+    __shared__ int s_inv[kP2GThreads / 64];   // a wave saw a bin out of index order
+    // the colour's particle indices, cell by cell in the order they are folded
+    __shared__ int s_cmp[kColourCellsMax * CAP];
+    // staged particles of one batch, paired for 16-byte LDS reads; slot s is
+    // stored at s ^ ((s >> 4) & 15) so the cells folded side by side read
+    // different LDS banks
     __shared__ double2 s_xy[kStageMax];     // local coordinates xl, yl
     __shared__ double2 s_msy[kStageMax];    // m, -sxy * v
     __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
@@ -475,10 +478,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             cnt_next[rr * g.Ne + cc] = 0;
         }
     }
-    // Bin counts of the 17 x 9 cells around the tile, and the colours that
-    // hold any particle.  An empty colour is skipped: folding no particles
-    // leaves every node sum as it was.
-    int cbits = 0;
+    // Bin counts of the 17 x 9 cells around the tile.
     for (int cl = tid; cl < kRegCells; cl += kP2GThreads) {
         const int cc = cx0 + cl % kRegX, rr = cy0 + cl / kRegX;
         int n = 0;
@@ -487,29 +487,62 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             n = n < CAP ? n : CAP;
         }
         s_cnt[cl] = n;
-        if (n > 0) {
-            cbits |= 1 << (2 * (rr & 1) + (cc & 1));
-        }
-    }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        cbits |= __shfl_xor(cbits, d);
-    }
-    if ((tid & 63) == 0) {
-        s_wbits[tid >> 6] = cbits;
     }
     __syncthreads();
-    int cmask = 0;
-#pragma unroll
-    for (int w = 0; w < kP2GThreads / 64; ++w) {
-        cmask |= s_wbits[w];
-    }
-    cmask = __builtin_amdgcn_readfirstlane(cmask);
 
     // Colour c's cells: region column a = 2*ax + ca holds cell column cx0 + a
     // (cx0 is odd, so colour parity pc <-> ca = 1 - pc); likewise rows.
     auto ncx_of = [](int c) { return (c & 1) ? (kRegX + 1) / 2 : kRegX / 2; };
     auto ncy_of = [](int c) { return (c >> 1) ? (kRegY + 1) / 2 : kRegY / 2; };
+
+    {
+        // Wave w lays out colour w: list offsets (a scan of the counts) and
+        // the staging batches.  Offsets never decrease and one cell holds at
+        // most CAP <= kStageMax particles, so the cells fitting a batch form
+        // a nonempty run, found with one ballot.
+        const int w = tid >> 6, lane = tid & 63;
+        const int ncx = ncx_of(w), nq = ncx * ncy_of(w);
+        const int ca = (w & 1) ? 0 : 1, cb = (w >> 1) ? 0 : 1;
+        int n = 0;
+        if (lane < nq) {
+            n = s_cnt[(2 * (lane / ncx) + cb) * kRegX + 2 * (lane % ncx) + ca];
+        }
+        int incl = n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int up = __shfl_up(incl, d);
+            if (lane >= d) incl += up;
+        }
+        if (lane < nq) {
+            s_off[w][lane + 1] = incl;
+        }
+        int q0 = 0, base = 0, nbat = 0;
+        while (q0 < nq) {
+            const unsigned long long fit =
+                __ballot(lane >= q0 && lane < nq && incl - base <= kStageMax);
+            const int q1 = q0 + __popcll(fit);
+            ++nbat;
+            if (lane == 0) {
+                s_bq[w][nbat] = q1;
+            }
+            base = __shfl(incl, q1 - 1);
+            q0 = q1;
+        }
+        if (lane == 0) {
+            s_off[w][0] = 0;
+            s_bq[w][0] = 0;
+            s_nbat[w] = nbat;
+        }
+    }
+    __syncthreads();
+    // Colours that hold any particle; an empty colour is skipped (folding no
+    // particles leaves every node sum as it was).
+    int cmask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        cmask |= (s_off[c][ncx_of(c) * ncy_of(c)] > 0) << c;
+    }
+    cmask = __builtin_amdgcn_readfirstlane(cmask);
 
     // Issues the loads of colour c's bin entries (all slots at once).  Slots
     // past a bin's count read a slot of an in-grid cell and are discarded, so
@@ -544,6 +577,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     auto ld = [](__amdgpu_buffer_rsrc_t r, int off) {
         return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     };
+    auto swz = [](int s) { return s ^ ((s >> 4) & 15); };
 
     const int i = tx0 + (nt % kTileX);
     const int j = ty0 + (nt / kTileX);
@@ -567,74 +601,44 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         const int ncx = ncx_of(c), ncy = ncy_of(c);
         const int ca = pc ? 0 : 1, cb = pr ? 0 : 1;
         const int nq = ncx * ncy;
+        // The colour's bin entries into the compact list.  Bins are filled
+        // in lane order, so they are nearly always sorted by particle index;
+        // a wave that sees an entry above its successor (entry k + 1 of a
+        // bin is one lane up) raises a flag, and only then are bins sorted.
+        bool inv = false;
 #pragma unroll
         for (int u = 0; u < kListSlots; ++u) {
             const int idx = tid + u * kP2GThreads;
-            if (idx < kColourCellsMax * CAP && lst_reg[u] >= 0) {
-                s_lst[(idx / CAP) * kLstStride + idx % CAP] = lst_reg[u];
+            const int q = idx / CAP, k = idx % CAP;
+            const int cur = lst_reg[u];
+            const int nxt = __shfl_down(cur, 1);
+            inv |= k < CAP - 1 && nxt >= 0 && nxt < cur;
+            if (idx < kColourCellsMax * CAP && cur >= 0) {
+                s_cmp[s_off[c][q] + k] = cur;
             }
+        }
+        const bool winv = __ballot(inv) != 0;
+        if ((tid & 63) == 0) {
+            s_inv[tid >> 6] = winv;
         }
         __syncthreads();
-        if (tid < 64) {
-            // staging offsets: an exclusive scan of (count + 1) over the
-            // colour's cells in the first wave (the extra slot per cell
-            // staggers the cells' staged records across LDS banks), the
-            // owner cell of every staged slot, and the batches of whole
-            // cells that fit the staging arrays
-            int n = 0;
+        if (__builtin_amdgcn_readfirstlane(s_inv[0] | s_inv[1] | s_inv[2] | s_inv[3])) {
             if (tid < nq) {
-                n = s_cnt[(2 * (tid / ncx) + cb) * kRegX + 2 * (tid % ncx) + ca] + 1;
-            }
-            int incl = n;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int up = __shfl_up(incl, d);
-                if (tid >= d) incl += up;
-            }
-            if (tid < nq) {
-                s_off[tid + 1] = incl;
-                for (int t = incl - n; t < incl; ++t) {
-                    s_owner[t] = (unsigned char)tid;
+                // insertion sort of one cell's entries
+                int *lst = &s_cmp[s_off[c][tid]];
+                const int n = s_off[c][tid + 1] - s_off[c][tid];
+                for (int k = 1; k < n; ++k) {
+                    const int key = lst[k];
+                    int r = k - 1;
+                    while (r >= 0 && lst[r] > key) {
+                        lst[r + 1] = lst[r];
+                        --r;
+                    }
+                    lst[r + 1] = key;
                 }
             }
-            // incl rises by at least 1 per cell and a cell needs at most
-            // CAP + 1 <= kStageMax slots, so each batch is a nonempty run.
-            int q0 = 0, base = 0, nbat = 0;
-            while (q0 < nq) {
-                const unsigned long long fit =
-                    __ballot(tid >= q0 && tid < nq && incl - base <= kStageMax);
-                const int q1 = q0 + __popcll(fit);
-                ++nbat;
-                if (tid == 0) {
-                    s_bq[nbat] = q1;
-                }
-                base = __shfl(incl, q1 - 1);
-                q0 = q1;
-            }
-            if (tid == 0) {
-                s_off[0] = 0;
-                s_bq[0] = 0;
-                s_nbat = nbat;
-            }
+            __syncthreads();
         }
-        if (tid >= kP2GThreads - kColourCellsMax && tid - (kP2GThreads - kColourCellsMax) < nq) {
-            // sort one bin by particle index (insertion sort; bins arrive
-            // nearly sorted because lanes of a wave insert in lane order)
-            const int q = tid - (kP2GThreads - kColourCellsMax);
-            const int a = 2 * (q % ncx) + ca, b = 2 * (q / ncx) + cb;
-            const int n = s_cnt[b * kRegX + a];
-            int *lst = &s_lst[q * kLstStride];
-            for (int k = 1; k < n; ++k) {
-                const int key = lst[k];
-                int r = k - 1;
-                while (r >= 0 && lst[r] > key) {
-                    lst[r + 1] = lst[r];
-                    --r;
-                }
-                lst[r + 1] = key;
-            }
-        }
-        __syncthreads();
 
         // This node's cell of colour c, as an index into the colour's cells.
         int myq = -1, dx = 0, dy = 0;
@@ -648,13 +652,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
         }
 
-        // Stages the cells [qb, qe) of the colour (cell q's records at
-        // [s_off[q], s_off[q+1] - 1); slot s_off[q+1] - 1 is padding) and
-        // folds them into this node's chains.  The first batch also issues
-        // the next colour's bin loads, after its own particle loads.
+        // Stages the cells [qb, qe) of the colour and folds them into this
+        // node's chains.  The first batch also issues the next colour's bin
+        // loads, after its own particle loads.
         auto batch = [&](int qb, int qe, bool first) {
-            const int base = s_off[qb];
-            const int nb = s_off[qe] - base;
+            const int base = s_off[c][qb];
+            const int nb = s_off[c][qe] - base;
             int off[kStageSlots], sc[kStageSlots];
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
@@ -663,12 +666,18 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 sc[u] = -1;
                 if (s < nb) {
                     const int gi = base + s;
-                    const int lo = s_owner[gi];
-                    const int k = gi - s_off[lo];
-                    if (k < s_off[lo + 1] - s_off[lo] - 1) {   // not the padding slot
-                        off[u] = s_lst[lo * kLstStride + k] * (int)sizeof(double);
-                        sc[u] = lo;
+                    // the slot's cell: the last q in [qb, qe) with s_off[q] <= gi
+                    int lo = qb, hi = qe - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[c][mid] <= gi) {
+                            lo = mid;
+                        } else {
+                            hi = mid - 1;
+                        }
                     }
+                    off[u] = s_cmp[gi] * (int)sizeof(double);
+                    sc[u] = lo;
                 }
             }
             double px[kStageSlots], py[kStageSlots], pm[kStageSlots], pv[kStageSlots];
@@ -691,7 +700,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
-                const int s = tid + u * kP2GThreads;
+                const int s = swz(tid + u * kP2GThreads);
                 if (sc[u] >= 0) {
                     const int lo = sc[u];
                     const int cc = cx0 + 2 * (lo % ncx) + ca, rr = cy0 + 2 * (lo / ncx) + cb;
@@ -706,9 +715,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
             __syncthreads();
             if (myq >= qb && myq < qe) {
-                const int k1 = s_off[myq + 1] - 1 - base;
+                const int k1 = s_off[c][myq + 1] - base;
                 const double2 *s_mom = role == 0 ? s_pxs : s_pys;
-                for (int s = s_off[myq] - base; s < k1; ++s) {
+                for (int s0 = s_off[c][myq] - base; s0 < k1; ++s0) {
+                    const int s = swz(s0);
                     const double2 xy = s_xy[s], msy = s_msy[s], ps = s_mom[s];
                     const double X = dx ? xy.x : 1.0 - xy.x;
                     const double Y = dy ? xy.y : 1.0 - xy.y;
@@ -736,10 +746,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             }
             __syncthreads();
         };
-        const int nbat = s_nbat;
-        batch(0, s_bq[1], true);
+        const int nbat = s_nbat[c];
+        batch(0, s_bq[c][1], true);
         for (int bi = 1; bi < nbat; ++bi) {
-            batch(s_bq[bi], s_bq[bi + 1], false);
+            batch(s_bq[c][bi], s_bq[c][bi + 1], false);
         }
     }
     }
