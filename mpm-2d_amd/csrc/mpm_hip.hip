@@ -717,6 +717,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             if (myq >= qb && myq < qe) {
                 const int k1 = s_off[c][myq + 1] - base;
                 const double2 *s_mom = role == 0 ? s_pxs : s_pys;
+#pragma unroll 2
                 for (int s0 = s_off[c][myq] - base; s0 < k1; ++s0) {
                     const int s = swz(s0);
                     const double2 xy = s_xy[s], msy = s_msy[s], ps = s_mom[s];
@@ -839,10 +840,37 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         st->bx = 0.0;
         st->by = gravity(t, st->t0, g.g_mag, g.ramp);
     }
-    if (p >= np || !P.active[p]) {
+    if (p >= np) {
         return;
     }
+    // Every per-particle load is issued here, before any is used: one
+    // memory round trip for the particle, then one for its four nodes.
+    const int act = P.active[p];
     const int e = P.elem[p];
+    double x = P.x[p], y = P.y[p];
+    const double vx0 = P.vx[p], vy0 = P.vy[p];
+    const double ux0 = P.ux[p], uy0 = P.uy[p];
+    const double mp = P.m[p], v0 = P.v[p];
+    const bool stress_in = !(mode & kModeHostMaterial) || (mode & kModeNonlocal);
+    const bool state_in = !(mode & (kModeHostMaterial | kModeNonlocal | kModeIsolin));
+    double sxx0 = 0.0, sxy0 = 0.0, syy0 = 0.0, szz0 = 0.0, gp0 = 0.0;
+    if (finish && stress_in) {
+        sxx0 = P.txx[p];
+        sxy0 = P.txy[p];
+        syy0 = P.tyy[p];
+    }
+    if (finish && state_in) {
+        szz0 = P.tzz[p];
+        gp0 = P.gp[p];
+    }
+    double xn0 = 0.0, yn0 = 0.0;
+    if (!move) {
+        xn0 = P.xn[p];
+        yn0 = P.yn[p];
+    }
+    if (!act) {
+        return;
+    }
     if (e < 0) {
         // left the grid during the previous step: deactivated at the start
         // of this one (process_usl.c:660-665)
@@ -851,7 +879,6 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     }
     const int cc = e % g.Ne, rr = e / g.Ne;
     const double dt = g.dt;
-    double x = P.x[p], y = P.y[p];
     Shape sh;
     shape_functions(x, y, cc, rr, g, sh);
 
@@ -865,7 +892,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
 
     // move_particles_explicit_usl_split
     double ax = 0.0, ay = 0.0;
-    double vx = P.vx[p], vy = P.vy[p];
+    double vx = vx0, vy = vy0;
     double ux, uy;
     if (move) {
 #pragma unroll
@@ -887,8 +914,8 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         duy *= dt;
         x += dux;
         y += duy;
-        ux = P.ux[p] + dux;
-        uy = P.uy[p] + duy;
+        ux = ux0 + dux;
+        uy = uy0 + duy;
         if (!finish) {
             // MUSL: the second mapping needs the new velocities at the
             // step's start positions; the new positions wait in xn, yn
@@ -907,10 +934,10 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
             return;
         }
     } else {
-        x = P.xn[p];
-        y = P.yn[p];
-        ux = P.ux[p];
-        uy = P.uy[p];
+        x = xn0;
+        y = yn0;
+        ux = ux0;
+        uy = uy0;
     }
 
     // calculate_strainrate_split (SMEAR4 folds left to right)
@@ -926,20 +953,19 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     const double wxy = 0.5 * (dx_tdy - dy_tdx);
 
     // update_particle_densities_split
-    const double mp = P.m[p];
-    const double v = P.v[p] * mpm_exp(dt * (exx + eyy));
+    const double v = v0 * mpm_exp(dt * (exx + eyy));
 
     if (mode & kModeNonlocal) {
         // the stress update waits for the fluidity solve; keep what it needs
         double gfb, xsi;
-        nonlocal_bulk(P.txx[p], P.txy[p], P.tyy[p], mt, gfb, xsi);
+        nonlocal_bulk(sxx0, sxy0, syy0, mt, gfb, xsi);
         P.gf_bulk[p] = gfb;
         P.xisq_inv[p] = xsi;
         P.elem_prev[p] = e;
     } else if (mode & kModeIsolin) {
         // isolin calculate_stress_threaded (materialsrc/isolin.c:89-110) on
         // sxx, sxy, syy (held in the Txx, Txy, Tyy arrays)
-        const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p];
+        const double sxx = sxx0, sxy = sxy0, syy = syy0;
         const double trD = exx + eyy;
         double dsjxx = mt.lambda * trD + mt.two_G * exx;
         double dsjxy = mt.two_G * exy;
@@ -953,7 +979,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     } else if (mode & kModeDpRiMt) {
         // dp_ri_mt calculate_stress_threaded (materialsrc/dp_ri_mt.c:133-205)
         // on sxx, sxy, syy (Txx, Txy, Tyy arrays) and szz = state[1] (Tzz)
-        const double sxx = P.txx[p], sxy = P.txy[p], syy = P.tyy[p], szz = P.tzz[p];
+        const double sxx = sxx0, sxy = sxy0, syy = syy0, szz = szz0;
         const double trD = exx + eyy;
         double dsjxx = mt.lambda * trD + mt.two_G * exx;
         double dsjxy = mt.two_G * exy;
@@ -995,11 +1021,11 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         } else {
             nup_tau = 0.0;   // NaN pressure: stress left unchanged
         }
-        P.gp[p] = P.gp[p] + nup_tau * dt;
+        P.gp[p] = gp0 + nup_tau * dt;
         P.gdp[p] = nup_tau;
     } else if (!(mode & kModeHostMaterial)) {
-        double txx = P.txx[p], txy = P.txy[p], tyy = P.tyy[p], tzz = P.tzz[p];
-        double gp = P.gp[p], gdp;
+        double txx = sxx0, txy = sxy0, tyy = syy0, tzz = szz0;
+        double gp = gp0, gdp;
         g_local_mu2_update(exx, exy, eyy, wxy, mp, v, dt, mt, txx, txy, tyy, tzz, gp, gdp);
         P.txx[p] = txx;
         P.txy[p] = txy;
