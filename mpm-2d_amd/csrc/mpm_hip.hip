@@ -86,6 +86,7 @@ struct GridParams {
     int cap;            // bin capacity
     double h;
     double inv_h;       // grad_tent's 1.0 / h
+    int h_pow2;         // h is a power of two
     double dt;
     double g_mag, ramp;
     int bc_kind;
@@ -143,10 +144,18 @@ struct BatchTables {
 // WHICH_ELEMENT4 (process_usl.c:43-44).  An index at or beyond the element
 // count (an ulp-level rounding case the reference reads out of bounds for)
 // is treated as outside.
+// a / h.  When h is a power of two the quotient is exact, and so is the
+// product with 1 / h (a power of two too): the same value, for one
+// multiplication instead of the division sequence.
+__device__ __forceinline__ double div_h(double a, const GridParams &g)
+{
+    return g.h_pow2 ? a * g.inv_h : a / g.h;
+}
+
 __device__ __forceinline__ int which_element(double x, double y, const GridParams &g)
 {
     if (x < 1.0 && x >= 0.0 && y < 1.0 && y >= 0.0) {
-        double e = floor(x / g.h) + floor(y / g.h) * (double)g.Ne;
+        double e = floor(div_h(x, g)) + floor(div_h(y, g)) * (double)g.Ne;
         int ei = (int)e;
         return ei < g.ncells ? ei : -1;
     }
@@ -176,8 +185,8 @@ __device__ __forceinline__ void shape_functions(double x, double y, int cc, int 
 {
     const double xn = (double)cc * g.h;
     const double yn = (double)rr * g.h;
-    const double xl = snap_local((x - xn) / g.h);
-    const double yl = snap_local((y - yn) / g.h);
+    const double xl = snap_local(div_h(x - xn, g));
+    const double yl = snap_local(div_h(y - yn, g));
     const double ox = 1.0 - xl, oy = 1.0 - yl;
     sh.s[0] = ox * oy;
     sh.s[1] = xl * oy;
@@ -382,9 +391,10 @@ __device__ __forceinline__ double gravity(double t, double t0, double g_mag, dou
 // to member init - 1.
 template <bool BATCH>
 __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
-    Particles P, int *__restrict__ cnt, int *__restrict__ bkt, GridParams g,
+    const Particles *__restrict__ Pd, int *__restrict__ cnt, int *__restrict__ bkt, GridParams g,
     DevState *__restrict__ st, int np, int init, BatchTables bt)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if constexpr (BATCH) {
         const int mb = bt.block_member[blockIdx.x];
@@ -412,11 +422,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
 // or the force update.
 template <int CAP, bool BATCH, bool MUSL2 = false>
 __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
-    Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
     int tiles_x, int ntiles, int chunk, int np, BatchTables bt)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     // bin entries of one colour's cells held per thread between colours
     constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
     constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;
@@ -705,8 +716,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                     const int lo = sc[u];
                     const int cc = cx0 + 2 * (lo % ncx) + ca, rr = cy0 + 2 * (lo / ncx) + cb;
                     const double mp = pm[u], vp = pv[u];
-                    s_xy[s] = make_double2(snap_local((px[u] - (double)cc * g.h) / g.h),
-                                           snap_local((py[u] - (double)rr * g.h) / g.h));
+                    s_xy[s] = make_double2(snap_local(div_h(px[u] - (double)cc * g.h, g)),
+                                           snap_local(div_h(py[u] - (double)rr * g.h, g)));
                     // pdata = {m, x_t m, y_t m, ..}, stressdata = {-sxx v, -sxy v, -syy v}
                     s_msy[s] = make_double2(mp, -psy[u] * vp);
                     s_pxs[s] = make_double2(pvx[u] * mp, -psx[u] * vp);
@@ -717,15 +728,21 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             if (myq >= qb && myq < qe) {
                 const int k1 = s_off[c][myq + 1] - base;
                 const double2 *s_mom = role == 0 ? s_pxs : s_pys;
+                // tent(xl) = dx ? xl : 1 - xl as one exact fma (the product
+                // with +-1 is exact; adding -0 leaves any value unchanged),
+                // and grad_tent's sign folded into the 1 / h factor
+                const double ax = dx ? 1.0 : -1.0, cx = dx ? -0.0 : 1.0;
+                const double ay = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
+                const double hx = dx ? g.inv_h : -g.inv_h, hy = dy ? g.inv_h : -g.inv_h;
 #pragma unroll 2
                 for (int s0 = s_off[c][myq] - base; s0 < k1; ++s0) {
                     const int s = swz(s0);
                     const double2 xy = s_xy[s], msy = s_msy[s], ps = s_mom[s];
-                    const double X = dx ? xy.x : 1.0 - xy.x;
-                    const double Y = dy ? xy.y : 1.0 - xy.y;
+                    const double X = fma(ax, xy.x, cx);
+                    const double Y = fma(ay, xy.y, cy);
                     const double sf = X * Y;
-                    const double b1 = (dx ? Y : -Y) * g.inv_h;
-                    const double b2 = (dy ? X : -X) * g.inv_h;
+                    const double b1 = Y * hx;
+                    const double b2 = X * hy;
                     const double mp = msy.x;
                     if (role == 0) {
                         am += sf * mp;
@@ -809,10 +826,11 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
 
 template <bool BATCH>
 __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
-    Particles P, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
+    const Particles *__restrict__ Pd, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
     int *__restrict__ bkt_next, GridParams g, Material mt, DevState *__restrict__ st,
     int np, int mode, BatchTables bt)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int first = 0;
     if constexpr (BATCH) {
@@ -1094,9 +1112,10 @@ constexpr int kNlMaxBlocks = 2048;
 // so the sums are deterministic.
 template <int CAP>
 __global__ __launch_bounds__(kNlThreads) void k_nl_assemble(
-    Particles P, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     const NodeRec *__restrict__ nodes, NlSystem S, GridParams g, int tiles_x, int ntiles)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     constexpr int T = 16, R = T + 1;
     __shared__ int s_cnt[R * R];
     __shared__ int s_lst[R * R * CAP];
@@ -1599,9 +1618,10 @@ __global__ __launch_bounds__(kCgThreads) void k_nl_cg(NlSystem S, double rel_tol
 // Nonlocal fluidity at the particles and the explicit visco-plastic update
 // (g_nonlocal.c:145-301, solve_diffusion_part 538-564), every particle.
 __global__ __launch_bounds__(kG2PThreads) void k_nl_stress(
-    Particles P, const double *__restrict__ gnode, GridParams g, Material mt,
+    const Particles *__restrict__ Pd, const double *__restrict__ gnode, GridParams g, Material mt,
     const DevState *__restrict__ st, int np)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) {
         return;
@@ -1846,9 +1866,10 @@ __device__ __forceinline__ void block_sum_params(double v[kAdjParams], double *s
 }
 
 __global__ __launch_bounds__(kG2PThreads) void k_adj_g2p(
-    Particles P, const NodeRec *__restrict__ nodes, AdjState A, GridParams g, Material mt,
+    const Particles *__restrict__ Pd, const NodeRec *__restrict__ nodes, AdjState A, GridParams g, Material mt,
     int np)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     __shared__ double s_red[kAdjParams * (kG2PThreads / 64)];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double pb[kAdjParams] = {0, 0, 0, 0, 0, 0, 0};
@@ -1958,10 +1979,11 @@ __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, 
 }
 
 __global__ __launch_bounds__(256) void k_adj_nodes(
-    Particles P, const int *__restrict__ cnt, const int *__restrict__ bkt,
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt, const int *__restrict__ bkt,
     const NodeRec *__restrict__ nodes, const uint8_t *__restrict__ mask, AdjState A,
     GridParams g, const DevState *__restrict__ st, int np)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     const int nidx = blockIdx.x * blockDim.x + threadIdx.x;
     const int nn = g.N * g.N;
     if (nidx >= nn) return;
@@ -2027,8 +2049,9 @@ __global__ __launch_bounds__(256) void k_adj_nodes(
 }
 
 __global__ __launch_bounds__(kG2PThreads) void k_adj_p2g(
-    Particles P, AdjState A, GridParams g, const DevState *__restrict__ st, int np)
+    const Particles *__restrict__ Pd, AdjState A, GridParams g, const DevState *__restrict__ st, int np)
 {
+    const Particles &P = *Pd;   // field pointers, loaded as used
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np || !P.active[p] || P.elem[p] < 0) return;
     const int e = P.elem[p];
@@ -2150,6 +2173,7 @@ struct mpm_hip_ctx {
     Material mt;
     hipStream_t stream;
     Particles P;
+    Particles *dP;          // device copy of P: kernels load field pointers as used
     double *pfield[MPM_F_COUNT];
     NodeRec *nodes;
     uint8_t *mask;
@@ -2206,6 +2230,16 @@ static void bind_fields(mpm_hip_ctx *ctx)
     P.xisq_inv = f[MPM_F_XISQ_INV];
 }
 
+// Refreshes the device copy of a particle-array table (after an allocation
+// changed it; dst may not exist yet during set-up).
+static int upload_particles(Particles *dst, const Particles &src)
+{
+    if (dst && hipMemcpy(dst, &src, sizeof(Particles), hipMemcpyHostToDevice) != hipSuccess) {
+        return MPM_HIP_ERR_DEVICE;
+    }
+    return MPM_HIP_OK;
+}
+
 // Auxiliary per-particle outputs, allocated (zeroed) on first use.
 static int alloc_aux(mpm_hip_ctx *ctx)
 {
@@ -2218,7 +2252,7 @@ static int alloc_aux(mpm_hip_ctx *ctx)
         }
     }
     bind_fields(ctx);
-    return MPM_HIP_OK;
+    return upload_particles(ctx->dP, ctx->P);
 }
 
 // k_p2g_grid's grid: whole rounds of 8 chunks (one per XCD); workgroups
@@ -2242,17 +2276,17 @@ static void p2g_kernel_launch(mpm_hip_ctx *ctx)
     const dim3 grid(ctx->p2g_blocks), block(kP2GThreads);
     switch (ctx->g.cap) {
     case 16:
-        k_p2g_grid<16, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<16, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
     case 32:
-        k_p2g_grid<32, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<32, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
     default:
-        k_p2g_grid<64, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->P, ctx->cnt[cur],
+        k_p2g_grid<64, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
             ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
             ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
         break;
@@ -2272,15 +2306,15 @@ static int launch_nonlocal(mpm_hip_ctx *ctx, int bins)
     const dim3 ag(ctx->nl_ntiles), ab(kNlThreads);
     switch (ctx->g.cap) {
     case 16:
-        k_nl_assemble<16><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+        k_nl_assemble<16><<<ag, ab, 0, ctx->stream>>>(ctx->dP, ctx->cnt[bins], ctx->bkt[bins],
             ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
         break;
     case 32:
-        k_nl_assemble<32><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+        k_nl_assemble<32><<<ag, ab, 0, ctx->stream>>>(ctx->dP, ctx->cnt[bins], ctx->bkt[bins],
             ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
         break;
     default:
-        k_nl_assemble<64><<<ag, ab, 0, ctx->stream>>>(ctx->P, ctx->cnt[bins], ctx->bkt[bins],
+        k_nl_assemble<64><<<ag, ab, 0, ctx->stream>>>(ctx->dP, ctx->cnt[bins], ctx->bkt[bins],
             ctx->nodes, ctx->nl, ctx->g, ctx->nl_tiles_x, ctx->nl_ntiles);
         break;
     }
@@ -2291,7 +2325,7 @@ static int launch_nonlocal(mpm_hip_ctx *ctx, int bins)
     HIP_TRY(hipLaunchCooperativeKernel((const void *)k_nl_cg, dim3(ctx->cg_blocks),
                                        dim3(kCgThreads), args, 0, ctx->stream));
     k_nl_stress<<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->P, ctx->nl.x, ctx->g, ctx->mt, ctx->st, (int)ctx->np);
+        ctx->dP, ctx->nl.x, ctx->g, ctx->mt, ctx->st, (int)ctx->np);
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
 }
@@ -2315,13 +2349,13 @@ static int launch_g2p(mpm_hip_ctx *ctx)
         // particle update; second mapping of the momentum; strain rate,
         // volume, stress and rebinning from the re-mapped velocities
         k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-            ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+            ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
             (int)ctx->np, mode | kModeMuslMove, BatchTables{});
         p2g_kernel_launch<true>(ctx);
         mode |= kModeMuslFinish;
     }
     k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->P, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+        ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
         (int)ctx->np, mode, BatchTables{});
     HIP_TRY(hipGetLastError());
     ctx->cur = nxt;
@@ -2352,6 +2386,7 @@ struct mpm_hip_batch {
     DevState *st;
     double *pfield[MPM_F_EXX];
     Particles P;
+    Particles *dP;             // device copy of P
     NodeRec *nodes;
     int *cnt[2], *bkt[2];
     size_t np_total, nn_total, ncells_total;
@@ -2379,15 +2414,15 @@ static int batch_launch_p2g(mpm_hip_batch *b, int cur)
     const GridParams g0{};
     switch (b->cap) {
     case 16:
-        k_p2g_grid<16, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<16, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     case 32:
-        k_p2g_grid<32, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<32, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     default:
-        k_p2g_grid<64, true><<<grid, block, 0, b->stream>>>(b->P, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<64, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     }
@@ -2401,7 +2436,7 @@ static int batch_launch_g2p(mpm_hip_batch *b, int cur)
     const GridParams g0{};
     const Material m0{};
     k_g2p_stress<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
-        b->P, b->nodes, b->cnt[nxt], b->bkt[nxt], g0, m0, nullptr, 0, 0, batch_tables(b));
+        b->dP, b->nodes, b->cnt[nxt], b->bkt[nxt], g0, m0, nullptr, 0, 0, batch_tables(b));
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
 }
@@ -2457,7 +2492,7 @@ static int batch_rebin(mpm_hip_batch *b, int init)
     b->cur = 0;
     const GridParams g0{};
     k_bin_particles<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
-        b->P, b->cnt[0], b->bkt[0], g0, nullptr, 0, init, batch_tables(b));
+        b->dP, b->cnt[0], b->bkt[0], g0, nullptr, 0, init, batch_tables(b));
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
 }
@@ -2470,7 +2505,7 @@ static int rebin(mpm_hip_ctx *ctx, int init)
     if (ctx->np) {
         const size_t nb = (ctx->np + kG2PThreads - 1) / kG2PThreads;
         k_bin_particles<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-            ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g, ctx->st, (int)ctx->np, init,
+            ctx->dP, ctx->cnt[0], ctx->bkt[0], ctx->g, ctx->st, (int)ctx->np, init,
             BatchTables{});
         HIP_TRY(hipGetLastError());
     }
@@ -2489,6 +2524,10 @@ static int params_from_config(const mpm_hip_config *cfg, int cap, GridParams &g,
     g.cap = cap;
     g.h = cfg->h;
     g.inv_h = 1.0 / cfg->h;
+    {
+        int ex = 0;
+        g.h_pow2 = frexp(cfg->h, &ex) == 0.5;
+    }
     g.dt = cfg->dt;
     g.g_mag = cfg->g_mag;
     g.ramp = cfg->ramp_time;
@@ -2730,7 +2769,12 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
         ALLOC_OR_FAIL(ctx->bkt[b], (size_t)g.ncells * cap * sizeof(int));
     }
     ALLOC_OR_FAIL(ctx->st, sizeof(DevState));
+    ALLOC_OR_FAIL(ctx->dP, sizeof(Particles));
 #undef ALLOC_OR_FAIL
+    if (upload_particles(ctx->dP, P) != MPM_HIP_OK) {
+        mpm_hip_destroy(ctx);
+        return MPM_HIP_ERR_DEVICE;
+    }
 
     if (hipMemsetAsync(ctx->mask, 0, nn, ctx->stream) != hipSuccess
         || hipMemsetAsync(ctx->nodes, 0, nn * sizeof(NodeRec), ctx->stream) != hipSuccess
@@ -3059,6 +3103,7 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx)
         if (S.barrier) (void)hipFree(S.barrier);
     }
     if (ctx->P.active) (void)hipFree(ctx->P.active);
+    if (ctx->dP) (void)hipFree(ctx->dP);
     if (ctx->nodes) (void)hipFree(ctx->nodes);
     if (ctx->mask) (void)hipFree(ctx->mask);
     for (int b = 0; b < 2; ++b) {
@@ -3192,7 +3237,12 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     BALLOC(b->st, (size_t)num_members * sizeof(DevState));
     BALLOC(b->tile_member, (size_t)b->ntiles * sizeof(int));
     BALLOC(b->block_member, (size_t)b->g2p_blocks * sizeof(int));
+    BALLOC(b->dP, sizeof(Particles));
 #undef BALLOC
+    if (upload_particles(b->dP, P) != MPM_HIP_OK) {
+        mpm_hip_batch_destroy(b);
+        return MPM_HIP_ERR_DEVICE;
+    }
 
     int *tm = (int *)malloc(sizeof(int) * (size_t)(b->ntiles ? b->ntiles : 1));
     int *bm = (int *)malloc(sizeof(int) * (size_t)b->g2p_blocks);
@@ -3533,7 +3583,7 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
         if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
     void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->cnt[0], b->cnt[1], b->bkt[0],
-                    b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member};
+                    b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member, b->dP};
     for (void *a : arrs) {
         if (a) (void)hipFree(a);
     }
@@ -3602,7 +3652,7 @@ static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, De
         ctx->cur = 0;
         if (a->np) {
             k_bin_particles<false><<<dim3((unsigned)a->g2p_blocks), dim3(kG2PThreads), 0,
-                                     ctx->stream>>>(ctx->P, ctx->cnt[0], ctx->bkt[0], ctx->g,
+                                     ctx->stream>>>(ctx->dP, ctx->cnt[0], ctx->bkt[0], ctx->g,
                                                     ctx->st, (int)a->np, 0, BatchTables{});
             HIP_TRY(hipGetLastError());
         }
@@ -3619,14 +3669,14 @@ static int adj_reverse_step(mpm_hip_adjoint *a)
     const int np = (int)a->np;
     const int nn = ctx->g.N * ctx->g.N;
     k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->P, ctx->nodes, a->A, ctx->g, ctx->mt, np);
+        ctx->dP, ctx->nodes, a->A, ctx->g, ctx->mt, np);
     k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->g.ncells, ctx->g.cap);
     k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        ctx->P, ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->nodes, ctx->mask, a->A, ctx->g,
+        ctx->dP, ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->nodes, ctx->mask, a->A, ctx->g,
         ctx->st, np);
     k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->P, a->A, ctx->g, ctx->st, np);
+        ctx->dP, a->A, ctx->g, ctx->st, np);
     k_adj_reduce<<<dim3(1), dim3(256), 0, ctx->stream>>>(a->A.partials, a->g2p_blocks, a->acc);
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
