@@ -320,33 +320,39 @@ __device__ __forceinline__ void g_local_mu2_update(double exx, double exy, doubl
     gdp = nup_tau;
 }
 
-// Appends particle p to the bin of cell `key` when `valid`.  Lanes of a wave
-// that share a cell take their slots with one atomic per distinct cell (the
-// slot order inside a bin is irrelevant: readers sort the bin by index).
-// Must be reached by every lane of the wave that has not exited.
+// Appends particle p to the bin of cell `key` when `valid`.  Each run of
+// consecutive lanes with the same cell takes its slots with one atomic, so
+// a run's entries land in lane order (particles of a cell are mostly
+// consecutive in memory, and readers check the order and sort the rare bin
+// that needs it).  Must be reached by every lane of the wave that has not
+// exited.
 __device__ __forceinline__ void bin_insert(bool valid, int key, int p, int *cnt, int *bkt,
                                            int cap, int *status)
 {
-    unsigned long long pending = __ballot(valid);
-    const unsigned long long lane_bit = 1ull << __lane_id();
-    while (pending) {
-        const int leader = __builtin_ctzll(pending);
-        const int lkey = __shfl(key, leader);
-        const unsigned long long group = __ballot(valid && key == lkey) & pending;
-        int base = 0;
-        if (lane_bit == (1ull << leader)) {
-            base = atomicAdd(&cnt[lkey], __popcll(group));
-        }
-        base = __shfl(base, leader);
-        if (group & lane_bit) {
-            const int r = base + __popcll(group & (lane_bit - 1));
-            if (r < cap) {
-                bkt[(size_t)lkey * cap + r] = p;
-            } else {
-                atomicOr(status, kStatusBinOverflow);
-            }
-        }
-        pending &= ~group;
+    const int lane = __lane_id();
+    const int prev = __shfl_up(key, 1);
+    const unsigned long long vmask = __ballot(valid);
+    const bool prev_same = lane > 0 && ((vmask >> (lane - 1)) & 1ull) && prev == key;
+    const bool start = valid && !prev_same;
+    const unsigned long long starts = __ballot(start);
+    if (!valid) {
+        return;
+    }
+    // this lane's run: from the last start at or below it up to the next
+    // start or invalid lane above that start
+    const int s0 = 63 - __builtin_clzll(starts & ((2ull << lane) - 1ull));
+    const unsigned long long above = (starts | ~vmask) & ~((2ull << s0) - 1ull);
+    const int s1 = above ? __builtin_ctzll(above) : 64;
+    int base = 0;
+    if (start) {
+        base = atomicAdd(&cnt[key], s1 - s0);
+    }
+    base = __shfl(base, s0);
+    const int r = base + (lane - s0);
+    if (r < cap) {
+        bkt[(size_t)key * cap + r] = p;
+    } else {
+        atomicOr(status, kStatusBinOverflow);
     }
 }
 
