@@ -446,6 +446,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     __shared__ int s_off[4][kColourCellsMax + 1];
     __shared__ int s_bq[4][kMaxBatches + 1];
     __shared__ int s_nbat[4];
+    // per colour: the cell holding list slot 16 b (slots are looked up from
+    // there: one step for bins of 16 entries)
+    __shared__ unsigned char s_blk[4][(kColourCellsMax * CAP + 15) / 16];
     __shared__ int s_inv[kP2GThreads / 64];   // a wave saw a bin out of index order
     // the colour's particle indices, cell by cell in the order they are folded
     __shared__ int s_cmp[kColourCellsMax * CAP];
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // Two threads per node: role 0 folds the m, mx, fx chains, role 1 the
     // my, fy chains (each chain is one node field, summed in the reference's
     // order by exactly one thread).
-    const int role = tid / kTileNodes;
+    const int role = __builtin_amdgcn_readfirstlane(tid / kTileNodes);   // uniform per wave
     const int nt = tid % kTileNodes;
 
     // This tile owns cells (tx0.., ty0..): clear their next-step bin counts.
@@ -532,6 +535,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         }
         if (lane < nq) {
             s_off[w][lane + 1] = incl;
+            for (int blk = (incl - n + 15) >> 4; blk < (incl + 15) >> 4; ++blk) {
+                s_blk[w][blk] = (unsigned char)lane;
+            }
         }
         int q0 = 0, base = 0, nbat = 0;
         while (q0 < nq) {
@@ -683,15 +689,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 sc[u] = -1;
                 if (s < nb) {
                     const int gi = base + s;
-                    // the slot's cell: the last q in [qb, qe) with s_off[q] <= gi
-                    int lo = qb, hi = qe - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_off[c][mid] <= gi) {
-                            lo = mid;
-                        } else {
-                            hi = mid - 1;
-                        }
+                    // the slot's cell: the last q with s_off[q] <= gi
+                    int lo = s_blk[c][gi >> 4];
+                    while (s_off[c][lo + 1] <= gi) {
+                        ++lo;
                     }
                     off[u] = s_cmp[gi] * (int)sizeof(double);
                     sc[u] = lo;
