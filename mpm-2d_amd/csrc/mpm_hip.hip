@@ -320,35 +320,65 @@ __device__ __forceinline__ void g_local_mu2_update(double exx, double exy, doubl
     gdp = nup_tau;
 }
 
-// Appends particle p to the bin of cell `key` when `valid`.  Each run of
-// consecutive lanes with the same cell takes its slots with one atomic, so
-// a run's entries land in lane order (particles of a cell are mostly
-// consecutive in memory, and readers check the order and sort the rare bin
-// that needs it).  Must be reached by every lane of the wave that has not
-// exited.
+// Appends particle p to the bin of cell `key` when `valid`.  The lanes of a
+// wave that share a cell take consecutive slots in lane order with one
+// atomic, so a bin is sorted by particle index whenever its particles come
+// from one wave (the fold order readers need; they check and sort the rare
+// bin filled from several waves).  When every cell of the wave is one run
+// of consecutive lanes with cells rising from run to run (particles stored
+// cell by cell), the runs are found with two ballots; otherwise one leader
+// per distinct cell is picked in turn.  Must be reached by every lane of the
+// wave that has not exited.
 __device__ __forceinline__ void bin_insert(bool valid, int key, int p, int *cnt, int *bkt,
                                            int cap, int *status)
 {
     const int lane = __lane_id();
-    const int prev = __shfl_up(key, 1);
+    const unsigned long long lane_bit = 1ull << lane;
     const unsigned long long vmask = __ballot(valid);
+    const int prev = __shfl_up(key, 1);
     const bool prev_same = lane > 0 && ((vmask >> (lane - 1)) & 1ull) && prev == key;
     const bool start = valid && !prev_same;
     const unsigned long long starts = __ballot(start);
-    if (!valid) {
-        return;
+    // the run start below this lane's own, and its cell
+    const unsigned long long lower = starts & (lane_bit - 1ull);
+    const int ps = lower ? 63 - __builtin_clzll(lower) : lane;
+    const int pkey = __shfl(key, ps);
+    const bool rising = __ballot(start && lower && pkey >= key) == 0;
+    int base = 0, r = 0;
+    if (rising) {
+        if (!valid) {
+            return;
+        }
+        // this lane's run: from the last start at or below it up to the
+        // next start or invalid lane above that start
+        const int s0 = 63 - __builtin_clzll(starts & ((2ull << lane) - 1ull));
+        const unsigned long long above = (starts | ~vmask) & ~((2ull << s0) - 1ull);
+        const int s1 = above ? __builtin_ctzll(above) : 64;
+        if (start) {
+            base = atomicAdd(&cnt[key], s1 - s0);
+        }
+        base = __shfl(base, s0);
+        r = base + (lane - s0);
+    } else {
+        unsigned long long pending = vmask;
+        while (pending) {
+            const int leader = __builtin_ctzll(pending);
+            const int lkey = __shfl(key, leader);
+            const unsigned long long group = __ballot(valid && key == lkey) & pending;
+            int b = 0;
+            if (lane_bit == (1ull << leader)) {
+                b = atomicAdd(&cnt[lkey], __popcll(group));
+            }
+            b = __shfl(b, leader);
+            if (group & lane_bit) {
+                r = b + __popcll(group & (lane_bit - 1));
+            }
+            pending &= ~group;
+        }
+        if (!valid) {
+            return;
+        }
     }
-    // this lane's run: from the last start at or below it up to the next
-    // start or invalid lane above that start
-    const int s0 = 63 - __builtin_clzll(starts & ((2ull << lane) - 1ull));
-    const unsigned long long above = (starts | ~vmask) & ~((2ull << s0) - 1ull);
-    const int s1 = above ? __builtin_ctzll(above) : 64;
-    int base = 0;
-    if (start) {
-        base = atomicAdd(&cnt[key], s1 - s0);
-    }
-    base = __shfl(base, s0);
-    const int r = base + (lane - s0);
     if (r < cap) {
         bkt[(size_t)key * cap + r] = p;
     } else {
@@ -1974,6 +2004,25 @@ __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, 
     int n = cnt[c];
     n = n < cap ? n : cap;
     int *l = bkt + (size_t)c * cap;
+    // Bins are filled in lane order and are nearly always sorted already:
+    // check the first 16 entries from registers (every capacity holds 16)
+    // and sort in place only when needed.
+    const int4 *l4 = reinterpret_cast<const int4 *>(l);
+    int v[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int4 w = l4[k];
+        v[4 * k] = w.x;
+        v[4 * k + 1] = w.y;
+        v[4 * k + 2] = w.z;
+        v[4 * k + 3] = w.w;
+    }
+    bool sorted = true;
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        sorted &= !(k < n && v[k - 1] > v[k]);
+    }
+    if (sorted && n <= 16) return;
     for (int k = 1; k < n; ++k) {
         const int key = l[k];
         int r = k - 1;
@@ -2008,17 +2057,39 @@ __global__ __launch_bounds__(256) void k_adj_nodes(
         const int cell = rr * g.Ne + cc;
         int n = cnt[cell];
         n = n < g.cap ? n : g.cap;
-        for (int k = 0; k < n; ++k) {
-            const int p = bkt[(size_t)cell * g.cap + k];
-            Shape sh;
-            shape_functions(P.x[p], P.y[p], cc, rr, g, sh);
-            const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
-            Fx += S * A.coef[(size_t)0 * np + p];
-            Fy += S * A.coef[(size_t)1 * np + p];
-            Ux += S * A.coef[(size_t)2 * np + p];
-            Uy += S * A.coef[(size_t)3 * np + p];
-            Vx += b1 * A.coef[(size_t)4 * np + p] + b2 * A.coef[(size_t)5 * np + p];
-            Vy += b2 * A.coef[(size_t)6 * np + p] + b1 * A.coef[(size_t)7 * np + p];
+        const int *bin = bkt + (size_t)cell * g.cap;
+        // Particles in chunks of 4: the chunk's loads are all in flight
+        // together; the sums still run in bin order.
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            int pk[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                pk[t] = bin[k0 + t < n ? k0 + t : k0];
+            }
+            double px[4], py[4], cf[4][8];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                px[t] = P.x[pk[t]];
+                py[t] = P.y[pk[t]];
+#pragma unroll
+                for (int f = 0; f < 8; ++f) {
+                    cf[t][f] = A.coef[(size_t)f * np + pk[t]];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (k0 + t < n) {
+                    Shape sh;
+                    shape_functions(px[t], py[t], cc, rr, g, sh);
+                    const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
+                    Fx += S * cf[t][0];
+                    Fy += S * cf[t][1];
+                    Ux += S * cf[t][2];
+                    Uy += S * cf[t][3];
+                    Vx += b1 * cf[t][4] + b2 * cf[t][5];
+                    Vy += b2 * cf[t][6] + b1 * cf[t][7];
+                }
+            }
         }
     }
     const NodeRec nd = nodes[nidx];
