@@ -2023,6 +2023,29 @@ __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, 
         sorted &= !(k < n && v[k - 1] > v[k]);
     }
     if (sorted && n <= 16) return;
+    if (n <= 16) {
+        // odd-even transposition sort in registers; slots past n hold
+        // INT_MAX (they are unused)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            v[k] = k < n ? v[k] : INT_MAX;
+        }
+#pragma unroll
+        for (int phase = 0; phase < 16; ++phase) {
+#pragma unroll
+            for (int k = phase & 1; k + 1 < 16; k += 2) {
+                const int lo = min(v[k], v[k + 1]), hi = max(v[k], v[k + 1]);
+                v[k] = lo;
+                v[k + 1] = hi;
+            }
+        }
+        int4 *w4 = reinterpret_cast<int4 *>(l);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            w4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+        }
+        return;
+    }
     for (int k = 1; k < n; ++k) {
         const int key = l[k];
         int r = k - 1;
@@ -2040,58 +2063,70 @@ __global__ __launch_bounds__(256) void k_adj_nodes(
     GridParams g, const DevState *__restrict__ st, int np)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
-    const int nidx = blockIdx.x * blockDim.x + threadIdx.x;
+    // Four lanes per node, one per colour: lane c gathers the node's cell of
+    // colour c; the four partial sums are then added in colour order.
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nidx = t >> 2, c = t & 3;
     const int nn = g.N * g.N;
-    if (nidx >= nn) return;
+    const bool in = nidx < nn;
     const int i = nidx % g.N, j = nidx / g.N;
     double Fx = 0, Fy = 0, Ux = 0, Uy = 0, Vx = 0, Vy = 0;
-    // the node's cells in colour order, with the node's corner index in each
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int pc = c & 1, pr = c >> 1;
-        const int cc = i - ((i ^ pc) & 1);
-        const int rr = j - ((j ^ pr) & 1);
-        if (cc < 0 || rr < 0 || cc >= g.Ne || rr >= g.Ne) continue;
+    const int pc = c & 1, pr = c >> 1;
+    const int cc = i - ((i ^ pc) & 1);
+    const int rr = j - ((j ^ pr) & 1);
+    if (in && cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
         const int dx = i - cc, dy = j - rr;
         const int q = dy ? (dx ? 2 : 3) : (dx ? 1 : 0);
         const int cell = rr * g.Ne + cc;
         int n = cnt[cell];
         n = n < g.cap ? n : g.cap;
         const int *bin = bkt + (size_t)cell * g.cap;
-        // Particles in chunks of 4: the chunk's loads are all in flight
-        // together; the sums still run in bin order.
+        // particles in chunks of 4: the chunk's loads are all in flight
+        // together; the sums still run in bin order
         for (int k0 = 0; k0 < n; k0 += 4) {
             int pk[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                pk[t] = bin[k0 + t < n ? k0 + t : k0];
+            for (int u = 0; u < 4; ++u) {
+                pk[u] = bin[k0 + u < n ? k0 + u : k0];
             }
             double px[4], py[4], cf[4][8];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                px[t] = P.x[pk[t]];
-                py[t] = P.y[pk[t]];
+            for (int u = 0; u < 4; ++u) {
+                px[u] = P.x[pk[u]];
+                py[u] = P.y[pk[u]];
 #pragma unroll
                 for (int f = 0; f < 8; ++f) {
-                    cf[t][f] = A.coef[(size_t)f * np + pk[t]];
+                    cf[u][f] = A.coef[(size_t)f * np + pk[u]];
                 }
             }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (k0 + t < n) {
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u < n) {
                     Shape sh;
-                    shape_functions(px[t], py[t], cc, rr, g, sh);
+                    shape_functions(px[u], py[u], cc, rr, g, sh);
                     const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
-                    Fx += S * cf[t][0];
-                    Fy += S * cf[t][1];
-                    Ux += S * cf[t][2];
-                    Uy += S * cf[t][3];
-                    Vx += b1 * cf[t][4] + b2 * cf[t][5];
-                    Vy += b2 * cf[t][6] + b1 * cf[t][7];
+                    Fx += S * cf[u][0];
+                    Fy += S * cf[u][1];
+                    Ux += S * cf[u][2];
+                    Uy += S * cf[u][3];
+                    Vx += b1 * cf[u][4] + b2 * cf[u][5];
+                    Vy += b2 * cf[u][6] + b1 * cf[u][7];
                 }
             }
         }
     }
+    auto colour_sum = [&](double v) {
+        const int l0 = __lane_id() & ~3;
+        const double v1 = __shfl(v, l0 + 1), v2 = __shfl(v, l0 + 2), v3 = __shfl(v, l0 + 3);
+        return ((v + v1) + v2) + v3;
+    };
+    Fx = colour_sum(Fx);
+    Fy = colour_sum(Fy);
+    Ux = colour_sum(Ux);
+    Uy = colour_sum(Uy);
+    Vx = colour_sum(Vx);
+    Vy = colour_sum(Vy);
+    if (!in || c != 0) return;
     const NodeRec nd = nodes[nidx];
     const double m = nd.m;
     double mb = 0.0, pxb = 0.0, pyb = 0.0, fxb = 0.0, fyb = 0.0;
@@ -3750,7 +3785,7 @@ static int adj_reverse_step(mpm_hip_adjoint *a)
         ctx->dP, ctx->nodes, a->A, ctx->g, ctx->mt, np);
     k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->g.ncells, ctx->g.cap);
-    k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
+    k_adj_nodes<<<dim3((4 * nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
         ctx->dP, ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->nodes, ctx->mask, a->A, ctx->g,
         ctx->st, np);
     k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
@@ -3766,6 +3801,7 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out)
 {
     if (!cfg || !out || checkpoint_every <= 0 || cfg->material_kind != MPM_MAT_G_LOCAL_MU2
+        || cfg->N > 23170   // four gather lanes per node stay within int range
         || cfg->scheme != MPM_SCHEME_USL
         || (cfg->flags & MPM_HIP_HOST_MATERIAL)) {
         return MPM_HIP_ERR_INVALID;
