@@ -676,18 +676,30 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         }
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_inv[0] | s_inv[1] | s_inv[2] | s_inv[3])) {
-            if (tid < nq) {
-                // insertion sort of one cell's entries
-                int *lst = &s_cmp[s_off[c][tid]];
-                const int n = s_off[c][tid + 1] - s_off[c][tid];
-                for (int k = 1; k < n; ++k) {
-                    const int key = lst[k];
-                    int r = k - 1;
-                    while (r >= 0 && lst[r] > key) {
-                        lst[r + 1] = lst[r];
-                        --r;
+            // each entry's rank by particle index within its bin, then every
+            // entry written to its sorted place
+            int rank[kListSlots];
+#pragma unroll
+            for (int u = 0; u < kListSlots; ++u) {
+                const int idx = tid + u * kP2GThreads;
+                const int q = idx / CAP;
+                const int cur = lst_reg[u];
+                rank[u] = 0;
+                if (idx < kColourCellsMax * CAP && cur >= 0) {
+                    const int b0 = s_off[c][q], b1 = s_off[c][q + 1];
+                    int r = 0;
+                    for (int t = b0; t < b1; ++t) {
+                        r += s_cmp[t] < cur;
                     }
-                    lst[r + 1] = key;
+                    rank[u] = b0 + r;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kListSlots; ++u) {
+                const int idx = tid + u * kP2GThreads;
+                if (idx < kColourCellsMax * CAP && lst_reg[u] >= 0) {
+                    s_cmp[rank[u]] = lst_reg[u];
                 }
             }
             __syncthreads();
@@ -1995,22 +2007,18 @@ __global__ __launch_bounds__(kG2PThreads) void k_adj_g2p(
     }
 }
 
-// Sorts every cell's bin by particle index (one thread per cell).
-__global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, int *__restrict__ bkt,
-                                                   int ncells, int cap)
+// Sorts one bin of R <= cap entries in registers (odd-even transposition;
+// slots past n hold INT_MAX and are unused).  Returns false if n > R.
+template <int R>
+__device__ __forceinline__ bool sort_bin_in_registers(int *l, int n)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int n = cnt[c];
-    n = n < cap ? n : cap;
-    int *l = bkt + (size_t)c * cap;
-    // Bins are filled in lane order and are nearly always sorted already:
-    // check the first 16 entries from registers (every capacity holds 16)
-    // and sort in place only when needed.
-    const int4 *l4 = reinterpret_cast<const int4 *>(l);
-    int v[16];
+    if (n > R) {
+        return false;
+    }
+    int4 *l4 = reinterpret_cast<int4 *>(l);
+    int v[R];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < R / 4; ++k) {
         const int4 w = l4[k];
         v[4 * k] = w.x;
         v[4 * k + 1] = w.y;
@@ -2019,31 +2027,45 @@ __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, 
     }
     bool sorted = true;
 #pragma unroll
-    for (int k = 1; k < 16; ++k) {
+    for (int k = 1; k < R; ++k) {
         sorted &= !(k < n && v[k - 1] > v[k]);
     }
-    if (sorted && n <= 16) return;
-    if (n <= 16) {
-        // odd-even transposition sort in registers; slots past n hold
-        // INT_MAX (they are unused)
+    if (sorted) {
+        return true;
+    }
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            v[k] = k < n ? v[k] : INT_MAX;
+    for (int k = 0; k < R; ++k) {
+        v[k] = k < n ? v[k] : INT_MAX;
+    }
+#pragma unroll
+    for (int phase = 0; phase < R; ++phase) {
+#pragma unroll
+        for (int k = phase & 1; k + 1 < R; k += 2) {
+            const int lo = min(v[k], v[k + 1]), hi = max(v[k], v[k + 1]);
+            v[k] = lo;
+            v[k + 1] = hi;
         }
+    }
 #pragma unroll
-        for (int phase = 0; phase < 16; ++phase) {
-#pragma unroll
-            for (int k = phase & 1; k + 1 < 16; k += 2) {
-                const int lo = min(v[k], v[k + 1]), hi = max(v[k], v[k + 1]);
-                v[k] = lo;
-                v[k + 1] = hi;
-            }
-        }
-        int4 *w4 = reinterpret_cast<int4 *>(l);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            w4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
-        }
+    for (int k = 0; k < R / 4; ++k) {
+        l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    }
+    return true;
+}
+
+// Sorts every cell's bin by particle index (one thread per cell; the
+// adjoint's gather order).  Bins are filled in lane order and are mostly
+// sorted already.
+__global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, int *__restrict__ bkt,
+                                                   int ncells, int cap)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int n = cnt[c];
+    n = n < cap ? n : cap;
+    int *l = bkt + (size_t)c * cap;
+    if (n <= 16 ? sort_bin_in_registers<16>(l, n)
+                : (cap >= 32 && sort_bin_in_registers<32>(l, n))) {
         return;
     }
     for (int k = 1; k < n; ++k) {
