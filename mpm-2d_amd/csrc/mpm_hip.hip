@@ -651,9 +651,8 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             continue;
         }
         const int pc = c & 1, pr = c >> 1;
-        const int ncx = ncx_of(c), ncy = ncy_of(c);
+        const int ncx = ncx_of(c);
         const int ca = pc ? 0 : 1, cb = pr ? 0 : 1;
-        const int nq = ncx * ncy;
         // The colour's bin entries into the compact list.  Bins are filled
         // in lane order, so they are nearly always sorted by particle index;
         // a wave that sees an entry above its successor (entry k + 1 of a
@@ -1302,6 +1301,25 @@ __device__ __forceinline__ double block_sum(double v, double *s_red)
     return t;   // valid in every thread
 }
 
+__device__ __forceinline__ double block_max(double v, double *s_red)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        v = fmax(v, __shfl_down(v, d));
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        s_red[w] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        t = fmax(t, s_red[k]);
+    }
+    return t;   // valid in every thread
+}
+
 // Sum of the per-workgroup partials, same order in every workgroup.
 __device__ __forceinline__ double grid_total(const double *part, int nblocks, double *s_red)
 {
@@ -1481,99 +1499,152 @@ __global__ __launch_bounds__(kCgThreads) void k_nl_cg(NlSystem S, double rel_tol
         }
     }
     if (ntiles <= nb) {
-        // One tile per workgroup for the whole solve: each thread keeps its
-        // node's vectors and stencil row in registers; only the halo comes
-        // from memory, loaded right after each grid barrier together with
-        // the partial sums, so an iteration waits on memory once.
+        // One tile per workgroup for the whole solve, each thread keeping
+        // its node's vectors and stencil row in registers, and a
+        // preconditioner that acts inside each tile: M^-1 = a Chebyshev
+        // polynomial of odd degree kChebDegree in D^-1 K_tile (K restricted
+        // to the tile's nodes) times D^-1, on [lmax / 100, lmax] with lmax
+        // the tile's Gershgorin bound of D^-1/2 K_tile D^-1/2.  An odd-degree
+        // Chebyshev polynomial keeps M SPD for any positive spectrum, so CG
+        // stays valid; it needs no communication between tiles, and cuts
+        // the iteration count about threefold against Jacobi.  Standard PCG:
+        // two grid barriers per iteration (p.Kp, then r.z with r.r, which
+        // also publishes z for the neighbours' halo of p).
+        constexpr int kChebDegree = 9;
+        __shared__ double s_t[2][E * E];   // preconditioner iterates, zero ring
         const bool have_tile = (int)blockIdx.x < ntiles;
         const int x0 = have_tile ? ((int)blockIdx.x % tiles_x) * kCgTile : 0;
         const int y0 = have_tile ? ((int)blockIdx.x / tiles_x) * kCgTile : 0;
         const int ix = x0 + lx, iy = y0 + ly;
         const bool own = have_tile && ix < N && iy < N;
         const int i = own ? iy * N + ix : 0;
+        const int me = (ly + 1) * E + lx + 1;
         double Kr[9];
-        double u = 0.0, w = 0.0, sv = 0.0, pv = 0.0, xv = 0.0, rv = 0.0, dv = 0.0;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             Kr[k] = own ? K[(size_t)k * nn + i] : 0.0;
         }
+        double xv = 0.0, rv = 0.0, dv = 0.0;
         if (own) {
-            u = S.u[i];
-            w = S.w[i];
             xv = S.x[i];
             rv = S.r[i];
             dv = dinv[i];
         }
-        // halo values of the iteration about to run (buffer 0 first)
         const int hxg = x0 + hx, hyg = y0 + hy;
         const bool hin = have_tile && hx != -2 && hxg >= 0 && hxg < N && hyg >= 0 && hyg < N;
         const int hj = hin ? hyg * N + hxg : 0;
-        double hu = 0.0, hw = 0.0, hs = 0.0, hd = 0.0;
-        if (hin) {
-            hu = S.u[hj];
-            hw = S.w[hj];
-            hs = S.s[hj];
-            hd = dinv[hj];
+        // the tile's Gershgorin bound, from D^-1 in LDS (zero outside the tile)
+        for (int k = threadIdx.x; k < E * E; k += blockDim.x) {
+            s_t[0][k] = 0.0;
+            s_t[1][k] = 0.0;
         }
-        while (it < max_iter && rr > stop) {
-            const int nxt = (it + 1) & 1;
-            double gl = 0.0, dll = 0.0, rrl = 0.0;
-            const double si = w + beta * sv;
-            const double ui = u - alpha * (dv * si);
-            double uh = 0.0;
-            if (hin) {
-                const double sj = hw + beta * hs;
-                uh = hu - alpha * (hd * sj);
+        __syncthreads();
+        s_t[0][me] = dv;
+        __syncthreads();
+        double rowg = 0.0;
+        if (own) {
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                for (int ox = -1; ox <= 1; ++ox) {
+                    rowg += fabs(Kr[(oy + 1) * 3 + (ox + 1)])
+                            * sqrt(dv * s_t[0][me + oy * E + ox]);
+                }
             }
-            s_u[(ly + 1) * E + lx + 1] = own ? ui : 0.0;
+        }
+        __syncthreads();
+        s_t[0][me] = 0.0;
+        const double lmax = block_max(rowg, s_red);
+        const double lmin = lmax / 100.0;
+        const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+        const double sigma = theta / delta;
+        // z = M^-1 r for this thread's node (every thread of the workgroup
+        // takes part: the tile's products go through LDS)
+        auto precond = [&](double r) {
+            double dd = dv * r / theta;
+            double z = dd;
+            double rho_old = 1.0 / sigma;
+            for (int k = 1; k < kChebDegree; ++k) {
+                double *t = s_t[k & 1];
+                if (own) {
+                    t[me] = z;
+                }
+                __syncthreads();
+                double kz = 0.0;
+#pragma unroll
+                for (int oy = -1; oy <= 1; ++oy) {
+#pragma unroll
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        kz += Kr[(oy + 1) * 3 + (ox + 1)] * t[me + oy * E + ox];
+                    }
+                }
+                const double rho = 1.0 / (2.0 * sigma - rho_old);
+                dd = rho * rho_old * dd + (2.0 * rho / delta) * (dv * (r - kz));
+                z += dd;
+                rho_old = rho;
+            }
+            return own ? z : 0.0;
+        };
+        double * const pz = part + 4 * kNlMaxBlocks;   // [rz0, pq, rz, rr] partials
+        double zv = precond(rv);
+        if (own) {
+            S.u[i] = zv;
+        }
+        double rz = block_sum(rv * zv, s_red);
+        if (threadIdx.x == 0) {
+            pz[blockIdx.x] = rz;
+        }
+        grid_barrier(S.barrier, target);
+        double hz = hin ? S.u[hj] : 0.0;
+        rz = grid_total(pz, nb, s_red);
+        double pv = zv, ph = hz;
+        while (it < max_iter && rr > stop) {
+            // q = K p, from p on the tile and its halo
+            s_u[me] = own ? pv : 0.0;
             if (hx != -2) {
-                s_u[(hy + 1) * E + hx + 1] = uh;
+                s_u[(hy + 1) * E + hx + 1] = ph;
             }
             __syncthreads();
+            double qv = 0.0;
             if (own) {
-                double wv = 0.0;
 #pragma unroll
                 for (int oy = -1; oy <= 1; ++oy) {
 #pragma unroll
                     for (int ox = -1; ox <= 1; ++ox) {
                         const int jx = ix + ox, jy = iy + oy;
                         if (jx >= 0 && jx < N && jy >= 0 && jy < N) {
-                            wv += Kr[(oy + 1) * 3 + (ox + 1)]
-                                  * s_u[(ly + 1 + oy) * E + lx + 1 + ox];
+                            qv += Kr[(oy + 1) * 3 + (ox + 1)] * s_u[me + oy * E + ox];
                         }
                     }
                 }
-                pv = u + beta * pv;
-                xv += alpha * pv;
-                rv = rv - alpha * si;
-                sv = si;
-                u = ui;
-                w = wv;
-                S.u[(size_t)nxt * nn + i] = u;
-                S.w[(size_t)nxt * nn + i] = w;
-                S.s[(size_t)nxt * nn + i] = sv;
-                gl += rv * u;
-                dll += w * u;
-                rrl += rv * rv;
             }
-            double *pt = part + (size_t)nxt * 4 * kNlMaxBlocks;
-            block_sum3(gl, dll, rrl, s_red);   // (its barriers also retire s_u's readers)
+            const double pq_b = block_sum(pv * qv, s_red);
             if (threadIdx.x == 0) {
-                pt[blockIdx.x] = gl;
-                pt[kNlMaxBlocks + blockIdx.x] = dll;
-                pt[2 * kNlMaxBlocks + blockIdx.x] = rrl;
+                pz[kNlMaxBlocks + blockIdx.x] = pq_b;
             }
             grid_barrier(S.barrier, target);
-            if (hin) {
-                hu = S.u[(size_t)nxt * nn + hj];
-                hw = S.w[(size_t)nxt * nn + hj];
-                hs = S.s[(size_t)nxt * nn + hj];
+            const double pq = grid_total(pz + kNlMaxBlocks, nb, s_red);
+            const double alpha = rz / pq;
+            xv += alpha * pv;
+            rv -= alpha * qv;
+            zv = precond(rv);
+            if (own) {
+                S.u[i] = zv;
             }
-            double gamma_new, delta_new;
-            grid_total3(pt, nb, gamma_new, delta_new, rr, s_red);
-            beta = gamma_new / gamma;
-            alpha = gamma_new / (delta_new - beta * gamma_new / alpha);
-            gamma = gamma_new;
+            double rzl = rv * zv, rrl = rv * rv, unused = 0.0;
+            block_sum3(rzl, rrl, unused, s_red);
+            if (threadIdx.x == 0) {
+                pz[2 * kNlMaxBlocks + blockIdx.x] = rzl;
+                pz[3 * kNlMaxBlocks + blockIdx.x] = rrl;
+            }
+            grid_barrier(S.barrier, target);
+            hz = hin ? S.u[hj] : 0.0;
+            const double rz_new = grid_total(pz + 2 * kNlMaxBlocks, nb, s_red);
+            rr = grid_total(pz + 3 * kNlMaxBlocks, nb, s_red);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            pv = zv + beta * pv;
+            ph = hz + beta * ph;
             ++it;
         }
         if (own) {
