@@ -40,6 +40,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "mpm_exp.h"
 #include "mpm_hip.h"
@@ -87,6 +88,7 @@ struct GridParams {
     double h;
     double inv_h;       // grad_tent's 1.0 / h
     int h_pow2;         // h is a power of two
+    int m_finite;       // every particle mass is finite (checked on the host)
     double dt;
     double g_mag, ramp;
     int bc_kind;
@@ -775,37 +777,58 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             __syncthreads();
             if (myq >= qb && myq < qe) {
                 const int k1 = s_off[c][myq + 1] - base;
-                const double2 *s_mom = role == 0 ? s_pxs : s_pys;
                 // tent(xl) = dx ? xl : 1 - xl as one exact fma (the product
                 // with +-1 is exact; adding -0 leaves any value unchanged),
                 // and grad_tent's sign folded into the 1 / h factor
                 const double ax = dx ? 1.0 : -1.0, cx = dx ? -0.0 : 1.0;
                 const double ay = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
                 const double hx = dx ? g.inv_h : -g.inv_h, hy = dy ? g.inv_h : -g.inv_h;
+                // one loop per role (the role is uniform per wave), so
+                // neither carries the other's selects
+                const int k0 = s_off[c][myq] - base;
+                // role 0 with no x body force: af starts at +0 and is only
+                // ever a sum, so it is never -0, and adding sf * (0 * m)
+                // (+-0 for a finite mass) leaves it unchanged -- the term
+                // is skipped
+                auto fold0 = [&](auto body_force) {
 #pragma unroll 2
-                for (int s0 = s_off[c][myq] - base; s0 < k1; ++s0) {
-                    const int s = swz(s0);
-                    const double2 xy = s_xy[s], msy = s_msy[s], ps = s_mom[s];
-                    const double X = fma(ax, xy.x, cx);
-                    const double Y = fma(ay, xy.y, cy);
-                    const double sf = X * Y;
-                    const double b1 = Y * hx;
-                    const double b2 = X * hy;
-                    const double mp = msy.x;
-                    if (role == 0) {
+                    for (int s0 = k0; s0 < k1; ++s0) {
+                        const int s = swz(s0);
+                        const double2 xy = s_xy[s], msy = s_msy[s], ps = s_pxs[s];
+                        const double X = fma(ax, xy.x, cx);
+                        const double Y = fma(ay, xy.y, cy);
+                        const double sf = X * Y;
+                        const double mp = msy.x;
                         am += sf * mp;
                         ap += sf * ps.x;
                         if (!MUSL2) {
-                            af += sf * (bb * mp);
-                            af += b1 * ps.y;       // -sxx v
-                            af += b2 * msy.y;      // -sxy v
+                            if (decltype(body_force)::value) {
+                                af += sf * (bb * mp);
+                            }
+                            af += (Y * hx) * ps.y;     // -sxx v
+                            af += (X * hy) * msy.y;    // -sxy v
                         }
+                    }
+                };
+                if (role == 0) {
+                    if (g.m_finite && bb == 0.0) {
+                        fold0(std::false_type{});
                     } else {
+                        fold0(std::true_type{});
+                    }
+                } else {
+#pragma unroll 2
+                    for (int s0 = k0; s0 < k1; ++s0) {
+                        const int s = swz(s0);
+                        const double2 xy = s_xy[s], msy = s_msy[s], ps = s_pys[s];
+                        const double X = fma(ax, xy.x, cx);
+                        const double Y = fma(ay, xy.y, cy);
+                        const double sf = X * Y;
                         ap += sf * ps.x;
                         if (!MUSL2) {
-                            af += sf * (bb * mp);
-                            af += b1 * msy.y;      // -sxy v
-                            af += b2 * ps.y;       // -syy v
+                            af += sf * (bb * msy.x);
+                            af += (Y * hx) * msy.y;    // -sxy v
+                            af += (X * hy) * ps.y;     // -syy v
                         }
                     }
                 }
@@ -2469,6 +2492,16 @@ static int p2g_grid_blocks(int ntiles, int chunk)
     return ((chunks + 7) / 8) * 8 * chunk;
 }
 
+static int all_finite(const double *a, size_t n)
+{
+    for (size_t k = 0; k < n; ++k) {
+        if (!std::isfinite(a[k])) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
 static int use_device(const mpm_hip_ctx *ctx)
 {
     HIP_TRY(hipSetDevice(ctx->device));
@@ -3041,6 +3074,7 @@ int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
             }
         }
     }
+    ctx->g.m_finite = all_finite(m, ctx->np);
     DevState s0;
     memset(&s0, 0, sizeof s0);
     s0.t = ctx->cfg.t0;
@@ -3063,6 +3097,9 @@ int mpm_hip_set_field(mpm_hip_ctx *ctx, int field, const double *host)
     if (ctx->np) {
         HIP_TRY(hipMemcpyAsync(ctx->pfield[field], host, ctx->np * sizeof(double),
                                hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (field == MPM_F_M) {
+        ctx->g.m_finite = all_finite(host, ctx->np);
     }
     if (field == MPM_F_X || field == MPM_F_Y) {
         rc = rebin(ctx, 0);
