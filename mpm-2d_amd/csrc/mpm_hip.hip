@@ -466,9 +466,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     int tiles_x, int ntiles, int chunk, int np, BatchTables bt)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
-    // bin entries of one colour's cells held per thread between colours
-    constexpr int kListSlots = (kColourCellsMax * CAP + kP2GThreads - 1) / kP2GThreads;
-    constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;
+    constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;   // per thread
     constexpr int kMaxBatches = 8;   // 45 cells of at most 64 particles: <= 5 batches
     static_assert(kP2GThreads / 64 == 4, "one wave per colour scans the colour's cells");
 
@@ -482,8 +480,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // there: one step for bins of 16 entries)
     __shared__ unsigned char s_blk[4][(kColourCellsMax * CAP + 15) / 16];
     __shared__ int s_inv[kP2GThreads / 64];   // a wave saw a bin out of index order
-    // the colour's particle indices, cell by cell in the order they are folded
-    __shared__ int s_cmp[kColourCellsMax * CAP];
+    // a batch's particle indices in bin order (used only to rank the
+    // entries of bins that are out of index order)
+    __shared__ int s_cmp[kStageSlots * kP2GThreads];
     // staged particles of one batch, paired for 16-byte LDS reads; slot s is
     // stored at s ^ ((s >> 4) & 15) so the cells folded side by side read
     // different LDS banks
@@ -599,27 +598,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     }
     cmask = __builtin_amdgcn_readfirstlane(cmask);
 
-    // Issues the loads of colour c's bin entries (all slots at once).  Slots
-    // past a bin's count read a slot of an in-grid cell and are discarded, so
-    // no branch separates the loads from the later ones the colour issues.
-    int lst_reg[kListSlots];
-    auto fetch_lists = [&](int c) {
-        const int ncx = ncx_of(c), nq = ncx * ncy_of(c);
-        const int ca = (c & 1) ? 0 : 1, cb = (c >> 1) ? 0 : 1;
-#pragma unroll
-        for (int u = 0; u < kListSlots; ++u) {
-            const int idx = tid + u * kP2GThreads;
-            const int q = idx / CAP, k = idx % CAP;
-            const int qq = q < nq ? q : nq - 1;
-            const int a = 2 * (qq % ncx) + ca, b = 2 * (qq / ncx) + cb;
-            const bool valid = q < nq && k < s_cnt[b * kRegX + a];
-            const int cc = min(max(cx0 + a, 0), g.Ne - 1);
-            const int rr = min(max(cy0 + b, 0), g.Ne - 1);
-            const int v = bkt_cur[(size_t)(rr * g.Ne + cc) * CAP + k];
-            lst_reg[u] = valid ? v : -1;
-        }
-    };
-
     // Particle fields as raw buffers: a load past the end returns 0 without
     // a memory request, which idle staging slots use instead of a branch.
     const int pbytes = np * (int)sizeof(double);
@@ -634,6 +612,62 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     };
     auto swz = [](int s) { return s ^ ((s >> 4) & 15); };
 
+    // Staging slot u of a batch holds record s = tid + u * kP2GThreads of the
+    // colour's list: entry k of the bin of cell q, where s_off[c][q] <= base
+    // + s (the cell comes from the block-owner map).  The bin's next entry is
+    // loaded too: bins are filled in lane order and are nearly always sorted
+    // by particle index, and the pair shows the exceptions.  Slots past the
+    // batch read a slot of an in-grid cell and are discarded, so no branch
+    // separates these loads from the particle loads around them.
+    struct Slots {
+        int p[kStageSlots];     // bin entry (raw load)
+        int pn[kStageSlots];    // the bin's next entry (raw load)
+        int q[kStageSlots];     // cell, -1 for an idle slot
+        int more;               // bit u: the bin has an entry after this one
+    };
+    auto fetch = [&](int c, int bi, Slots &R) {
+        const int ncx = ncx_of(c);
+        const int ca = (c & 1) ? 0 : 1, cb = (c >> 1) ? 0 : 1;
+        const int base = s_off[c][s_bq[c][bi]];
+        const int nb = s_off[c][s_bq[c][bi + 1]] - base;
+        R.more = 0;
+#pragma unroll
+        for (int u = 0; u < kStageSlots; ++u) {
+            const int s = tid + u * kP2GThreads;
+            const bool valid = s < nb;
+            const int gi = base + s;
+            int q = s_bq[c][bi];   // idle slots: any cell (the address is clamped)
+            if (valid) {
+                q = s_blk[c][gi >> 4];
+                while (s_off[c][q + 1] <= gi) {
+                    ++q;
+                }
+            }
+            const int k = valid ? gi - s_off[c][q] : 0;
+            const bool more = valid && gi + 1 < s_off[c][q + 1];
+            const int cc = min(max(cx0 + 2 * (q % ncx) + ca, 0), g.Ne - 1);
+            const int rr = min(max(cy0 + 2 * (q / ncx) + cb, 0), g.Ne - 1);
+            const int *bin = bkt_cur + (size_t)(rr * g.Ne + cc) * CAP;
+            R.p[u] = bin[k];
+            R.pn[u] = bin[more ? k + 1 : k];
+            R.q[u] = valid ? q : -1;
+            R.more |= (int)more << u;
+        }
+    };
+    // Raises the wave's flag if a bin entry of the batch exceeds its
+    // successor (waits for the batch's bin loads).
+    auto flag_order = [&](const Slots &R) {
+        bool inv = false;
+#pragma unroll
+        for (int u = 0; u < kStageSlots; ++u) {
+            inv |= ((R.more >> u) & 1) && R.pn[u] < R.p[u];
+        }
+        const bool winv = __ballot(inv) != 0;
+        if ((tid & 63) == 0) {
+            s_inv[tid >> 6] = winv;
+        }
+    };
+
     const int i = tx0 + (nt % kTileX);
     const int j = ty0 + (nt / kTileX);
     const bool live = i < g.N && j < g.N;
@@ -641,70 +675,19 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     double am = 0.0, ap = 0.0, af = 0.0;             // role 0: m, mx, fx; role 1: -, my, fy
 
     if (cmask != 0) {
-    fetch_lists(0);
+    Slots Rn;
+    fetch(0, 0, Rn);
+    flag_order(Rn);
+    __syncthreads();
     // Colours in the reference's order; colour = 2 * (row % 2) + (col % 2).
     // Unrolled, so each colour's cell geometry is a compile-time constant.
+    // Every colour is visited once the tile holds particles (an empty one
+    // is a single batch of idle slots).
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        if (!((cmask >> c) & 1)) {
-            if (c < 3) {
-                fetch_lists(c + 1);
-            }
-            continue;
-        }
         const int pc = c & 1, pr = c >> 1;
         const int ncx = ncx_of(c);
         const int ca = pc ? 0 : 1, cb = pr ? 0 : 1;
-        // The colour's bin entries into the compact list.  Bins are filled
-        // in lane order, so they are nearly always sorted by particle index;
-        // a wave that sees an entry above its successor (entry k + 1 of a
-        // bin is one lane up) raises a flag, and only then are bins sorted.
-        bool inv = false;
-#pragma unroll
-        for (int u = 0; u < kListSlots; ++u) {
-            const int idx = tid + u * kP2GThreads;
-            const int q = idx / CAP, k = idx % CAP;
-            const int cur = lst_reg[u];
-            const int nxt = __shfl_down(cur, 1);
-            inv |= k < CAP - 1 && nxt >= 0 && nxt < cur;
-            if (idx < kColourCellsMax * CAP && cur >= 0) {
-                s_cmp[s_off[c][q] + k] = cur;
-            }
-        }
-        const bool winv = __ballot(inv) != 0;
-        if ((tid & 63) == 0) {
-            s_inv[tid >> 6] = winv;
-        }
-        __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(s_inv[0] | s_inv[1] | s_inv[2] | s_inv[3])) {
-            // each entry's rank by particle index within its bin, then every
-            // entry written to its sorted place
-            int rank[kListSlots];
-#pragma unroll
-            for (int u = 0; u < kListSlots; ++u) {
-                const int idx = tid + u * kP2GThreads;
-                const int q = idx / CAP;
-                const int cur = lst_reg[u];
-                rank[u] = 0;
-                if (idx < kColourCellsMax * CAP && cur >= 0) {
-                    const int b0 = s_off[c][q], b1 = s_off[c][q + 1];
-                    int r = 0;
-                    for (int t = b0; t < b1; ++t) {
-                        r += s_cmp[t] < cur;
-                    }
-                    rank[u] = b0 + r;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < kListSlots; ++u) {
-                const int idx = tid + u * kP2GThreads;
-                if (idx < kColourCellsMax * CAP && lst_reg[u] >= 0) {
-                    s_cmp[rank[u]] = lst_reg[u];
-                }
-            }
-            __syncthreads();
-        }
 
         // This node's cell of colour c, as an index into the colour's cells.
         int myq = -1, dx = 0, dy = 0;
@@ -717,29 +700,52 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 dy = j - rr;
             }
         }
+        // tent(xl) = dx ? xl : 1 - xl as one exact fma (the product with +-1
+        // is exact; adding -0 leaves any value unchanged), and grad_tent's
+        // sign folded into the 1 / h factor
+        const double ax = dx ? 1.0 : -1.0, cx = dx ? -0.0 : 1.0;
+        const double ay = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
+        const double hx = dx ? g.inv_h : -g.inv_h, hy = dy ? g.inv_h : -g.inv_h;
 
-        // Stages the cells [qb, qe) of the colour and folds them into this
-        // node's chains.  The first batch also issues the next colour's bin
-        // loads, after its own particle loads.
-        auto batch = [&](int qb, int qe, bool first) {
+        const int nbat = s_nbat[c];
+        for (int bi = 0; bi < nbat; ++bi) {
+            const Slots Rc = Rn;
+            const int qb = s_bq[c][bi], qe = s_bq[c][bi + 1];
             const int base = s_off[c][qb];
-            const int nb = s_off[c][qe] - base;
-            int off[kStageSlots], sc[kStageSlots];
+            // staged position of each slot's record: its own slot when the
+            // bins are in order, else its cell's offset + its rank by
+            // particle index within the bin (ranked through LDS)
+            int dst[kStageSlots];
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
-                const int s = tid + u * kP2GThreads;
-                off[u] = -(int)sizeof(double);   // past the end: idle slot
-                sc[u] = -1;
-                if (s < nb) {
-                    const int gi = base + s;
-                    // the slot's cell: the last q with s_off[q] <= gi
-                    int lo = s_blk[c][gi >> 4];
-                    while (s_off[c][lo + 1] <= gi) {
-                        ++lo;
+                dst[u] = tid + u * kP2GThreads;
+            }
+            if (__builtin_amdgcn_readfirstlane(s_inv[0] | s_inv[1] | s_inv[2] | s_inv[3])) {
+#pragma unroll
+                for (int u = 0; u < kStageSlots; ++u) {
+                    if (Rc.q[u] >= 0) {
+                        s_cmp[dst[u]] = Rc.p[u];
                     }
-                    off[u] = s_cmp[gi] * (int)sizeof(double);
-                    sc[u] = lo;
                 }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < kStageSlots; ++u) {
+                    if (Rc.q[u] >= 0) {
+                        const int b0 = s_off[c][Rc.q[u]] - base;
+                        const int b1 = s_off[c][Rc.q[u] + 1] - base;
+                        int rank = 0;
+                        for (int t = b0; t < b1; ++t) {
+                            rank += s_cmp[t] < Rc.p[u];
+                        }
+                        dst[u] = b0 + rank;
+                    }
+                }
+            }
+            // the batch's particle loads, then the bins of the next batch
+            int off[kStageSlots];
+#pragma unroll
+            for (int u = 0; u < kStageSlots; ++u) {
+                off[u] = Rc.q[u] >= 0 ? Rc.p[u] * (int)sizeof(double) : -(int)sizeof(double);
             }
             double px[kStageSlots], py[kStageSlots], pm[kStageSlots], pv[kStageSlots];
             double pvx[kStageSlots], pvy[kStageSlots];
@@ -756,40 +762,39 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 psy[u] = ld(r_sy, off[u]);
                 psz[u] = ld(r_sz, off[u]);
             }
-            if (first && c < 3) {
-                fetch_lists(c + 1);   // in flight while this colour is staged and folded
+            const bool last = bi + 1 == nbat;
+            if (!last) {
+                fetch(c, bi + 1, Rn);
+            } else if (c < 3) {
+                fetch(c + 1, 0, Rn);
             }
 #pragma unroll
             for (int u = 0; u < kStageSlots; ++u) {
-                const int s = swz(tid + u * kP2GThreads);
-                if (sc[u] >= 0) {
-                    const int lo = sc[u];
+                if (Rc.q[u] >= 0) {
+                    const int lo = Rc.q[u];
                     const int cc = cx0 + 2 * (lo % ncx) + ca, rr = cy0 + 2 * (lo / ncx) + cb;
                     const double mp = pm[u], vp = pv[u];
-                    s_xy[s] = make_double2(snap_local(div_h(px[u] - (double)cc * g.h, g)),
-                                           snap_local(div_h(py[u] - (double)rr * g.h, g)));
+                    const int sl = swz(dst[u]);
+                    s_xy[sl] = make_double2(snap_local(div_h(px[u] - (double)cc * g.h, g)),
+                                            snap_local(div_h(py[u] - (double)rr * g.h, g)));
                     // pdata = {m, x_t m, y_t m, ..}, stressdata = {-sxx v, -sxy v, -syy v}
-                    s_msy[s] = make_double2(mp, -psy[u] * vp);
-                    s_pxs[s] = make_double2(pvx[u] * mp, -psx[u] * vp);
-                    s_pys[s] = make_double2(pvy[u] * mp, -psz[u] * vp);
+                    s_msy[sl] = make_double2(mp, -psy[u] * vp);
+                    s_pxs[sl] = make_double2(pvx[u] * mp, -psx[u] * vp);
+                    s_pys[sl] = make_double2(pvy[u] * mp, -psz[u] * vp);
                 }
+            }
+            if (!last || c < 3) {
+                flag_order(Rn);
             }
             __syncthreads();
             if (myq >= qb && myq < qe) {
-                const int k1 = s_off[c][myq + 1] - base;
-                // tent(xl) = dx ? xl : 1 - xl as one exact fma (the product
-                // with +-1 is exact; adding -0 leaves any value unchanged),
-                // and grad_tent's sign folded into the 1 / h factor
-                const double ax = dx ? 1.0 : -1.0, cx = dx ? -0.0 : 1.0;
-                const double ay = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
-                const double hx = dx ? g.inv_h : -g.inv_h, hy = dy ? g.inv_h : -g.inv_h;
-                // one loop per role (the role is uniform per wave), so
-                // neither carries the other's selects
                 const int k0 = s_off[c][myq] - base;
+                const int k1 = s_off[c][myq + 1] - base;
                 // role 0 with no x body force: af starts at +0 and is only
                 // ever a sum, so it is never -0, and adding sf * (0 * m)
                 // (+-0 for a finite mass) leaves it unchanged -- the term
-                // is skipped
+                // is skipped.  One loop per role (the role is uniform per
+                // wave), so neither carries the other's selects.
                 auto fold0 = [&](auto body_force) {
 #pragma unroll 2
                     for (int s0 = k0; s0 < k1; ++s0) {
@@ -834,11 +839,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                 }
             }
             __syncthreads();
-        };
-        const int nbat = s_nbat[c];
-        batch(0, s_bq[c][1], true);
-        for (int bi = 1; bi < nbat; ++bi) {
-            batch(s_bq[c][bi], s_bq[c][bi + 1], false);
         }
     }
     }
