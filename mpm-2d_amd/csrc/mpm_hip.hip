@@ -783,9 +783,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                     s_pys[sl] = make_double2(pvy[u] * mp, -psz[u] * vp);
                 }
             }
-            if (!last || c < 3) {
-                flag_order(Rn);
-            }
             __syncthreads();
             if (myq >= qb && myq < qe) {
                 const int k0 = s_off[c][myq] - base;
@@ -837,6 +834,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                         }
                     }
                 }
+            }
+            // the next batch's order flags (its bin loads had the fold to land)
+            if (!last || c < 3) {
+                flag_order(Rn);
             }
             __syncthreads();
         }
