@@ -138,6 +138,10 @@ typedef struct mpm_hip_config {
     double cg_rel_tol;      /* g_nonlocal solve: stop at |r| <= tol |f| */
     int cg_max_iter;        /* g_nonlocal solve: iteration cap */
     int scheme;             /* mpm_hip_scheme */
+    int cg_max_blocks;      /* g_nonlocal solve: cap on its persistent
+                               workgroups, 0 = as many as fit (with fewer
+                               workgroups than 32x32-node tiles the solve
+                               takes its Jacobi-preconditioned path) */
 } mpm_hip_config;
 
 enum mpm_hip_scheme {

@@ -2863,6 +2863,7 @@ void mpm_hip_default_config(mpm_hip_config *cfg)
     cfg->nonlocal.c = 1e-2;
     cfg->cg_rel_tol = 1e-10;
     cfg->cg_max_iter = 20000;
+    cfg->cg_max_blocks = 0;
     cfg->scheme = MPM_SCHEME_USL;
     cfg->bin_capacity = 0;
     cfg->flags = 0;
@@ -2999,6 +3000,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
         long want = tx * tx;   // one workgroup per tile when they fit
         long have = (long)occ * prop.multiProcessorCount;
         long nbk = want < have ? want : have;
+        if (cfg->cg_max_blocks > 0 && nbk > cfg->cg_max_blocks) nbk = cfg->cg_max_blocks;
         ctx->cg_blocks = (int)(nbk < kNlMaxBlocks ? nbk : kNlMaxBlocks);
     }
     ALLOC_OR_FAIL(P.active, np * sizeof(int));

@@ -111,12 +111,12 @@ def test_uniform_state_gives_uniform_fluidity():
 # ---------------------------------------------------------------------------
 # GPU
 
-def _gpu_nonlocal(w, steps, tol=1e-14):
+def _gpu_nonlocal(w, steps, tol=1e-14, cg_max_blocks=0):
     cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
                             bc_kind=w.get("bc_kind", 0), bc_fp64=w.get("bc_fp64", ()),
                             bc_int=w.get("bc_int", ()), g_mag=w["g_mag"],
                             ramp_time=w["ramp_time"], nonlocal_material=w["nonlocal_material"],
-                            cg_rel_tol=tol, cg_max_iter=50000)
+                            cg_rel_tol=tol, cg_max_iter=50000, cg_max_blocks=cg_max_blocks)
     sim = pympm.Sim(cfg, w["particles"])
     sim.step(steps)
     sim.sync()
@@ -139,11 +139,15 @@ def test_gpu_uniform_state_gives_uniform_fluidity():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("steps,tol", [(1, 1e-12), (100, 1e-8)])
-def test_gpu_nonlocal_heap_matches_oracle(steps, tol):
+@pytest.mark.parametrize("steps,tol,blocks", [(1, 1e-12, 0), (100, 1e-8, 0), (1, 1e-12, 1),
+                                             (20, 1e-10, 1)])
+def test_gpu_nonlocal_heap_matches_oracle(steps, tol, blocks):
+    # blocks = 1: one persistent workgroup for the 2 x 2 tiles of the grid,
+    # which takes the solve's tiled, Jacobi-preconditioned path; blocks = 0:
+    # one workgroup per tile, the Chebyshev-preconditioned path
     w = workloads.heap(cells=32, seed=6)
-    print(f"case nonlocal heap_32 steps={steps}", flush=True)
-    sim = _gpu_nonlocal(w, steps)
+    print(f"case nonlocal heap_32 steps={steps} cg_max_blocks={blocks}", flush=True)
+    sim = _gpu_nonlocal(w, steps, cg_max_blocks=blocks)
     it, res = sim.solver_stats()
     assert res <= 1e-13
     o = pyoracle.from_workload(w, num_threads=4)
