@@ -529,7 +529,9 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             cnt_next[rr * g.Ne + cc] = 0;
         }
     }
-    // Bin counts of the 17 x 9 cells around the tile.
+    // Bin counts of the 17 x 9 cells around the tile; a tile whose region
+    // holds no particle skips to writing its (zero) node records.
+    int occupied = 0;
     for (int cl = tid; cl < kRegCells; cl += kP2GThreads) {
         const int cc = cx0 + cl % kRegX, rr = cy0 + cl / kRegX;
         int n = 0;
@@ -538,15 +540,16 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             n = n < CAP ? n : CAP;
         }
         s_cnt[cl] = n;
+        occupied |= n > 0;
     }
-    __syncthreads();
+    const bool any_particles = __syncthreads_or(occupied) != 0;
 
     // Colour c's cells: region column a = 2*ax + ca holds cell column cx0 + a
     // (cx0 is odd, so colour parity pc <-> ca = 1 - pc); likewise rows.
     auto ncx_of = [](int c) { return (c & 1) ? (kRegX + 1) / 2 : kRegX / 2; };
     auto ncy_of = [](int c) { return (c >> 1) ? (kRegY + 1) / 2 : kRegY / 2; };
 
-    {
+    if (any_particles) {
         // Wave w lays out colour w: list offsets (a scan of the counts) and
         // the staging batches.  Offsets never decrease and one cell holds at
         // most CAP <= kStageMax particles, so the cells fitting a batch form
@@ -589,14 +592,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         }
     }
     __syncthreads();
-    // Colours that hold any particle; an empty colour is skipped (folding no
-    // particles leaves every node sum as it was).
-    int cmask = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        cmask |= (s_off[c][ncx_of(c) * ncy_of(c)] > 0) << c;
-    }
-    cmask = __builtin_amdgcn_readfirstlane(cmask);
 
     // Particle fields as raw buffers: a load past the end returns 0 without
     // a memory request, which idle staging slots use instead of a branch.
@@ -674,7 +669,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const double bb = role == 0 ? st->bx : st->by;   // body force of this role's chain
     double am = 0.0, ap = 0.0, af = 0.0;             // role 0: m, mx, fx; role 1: -, my, fy
 
-    if (cmask != 0) {
+    if (any_particles) {
     Slots Rn;
     fetch(0, 0, Rn);
     flag_order(Rn);
