@@ -2097,24 +2097,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_adj_g2p(
     }
 }
 
-// Sorts one bin of R <= cap entries in registers (odd-even transposition;
-// slots past n hold INT_MAX and are unused).  Returns false if n > R.
+// Sorts the first n of R register values (odd-even transposition network;
+// the values past n are replaced by INT_MAX).  Returns whether the first n
+// were already sorted (and then leaves v alone).
 template <int R>
-__device__ __forceinline__ bool sort_bin_in_registers(int *l, int n)
+__device__ __forceinline__ bool sort_in_registers(int (&v)[R], int n)
 {
-    if (n > R) {
-        return false;
-    }
-    int4 *l4 = reinterpret_cast<int4 *>(l);
-    int v[R];
-#pragma unroll
-    for (int k = 0; k < R / 4; ++k) {
-        const int4 w = l4[k];
-        v[4 * k] = w.x;
-        v[4 * k + 1] = w.y;
-        v[4 * k + 2] = w.z;
-        v[4 * k + 3] = w.w;
-    }
     bool sorted = true;
 #pragma unroll
     for (int k = 1; k < R; ++k) {
@@ -2136,26 +2124,56 @@ __device__ __forceinline__ bool sort_bin_in_registers(int *l, int n)
             v[k + 1] = hi;
         }
     }
-#pragma unroll
-    for (int k = 0; k < R / 4; ++k) {
-        l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
-    }
-    return true;
+    return false;
 }
 
 // Sorts every cell's bin by particle index (one thread per cell; the
 // adjoint's gather order).  Bins are filled in lane order and are mostly
-// sorted already.
+// sorted already; bins of up to 32 entries are checked and sorted in
+// registers (the first 16 entries are loaded with the count).
 __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, int *__restrict__ bkt,
                                                    int ncells, int cap)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
+    int *l = bkt + (size_t)c * cap;
+    int4 *l4 = reinterpret_cast<int4 *>(l);
+    int v[32];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {   // every capacity holds 16 entries
+        const int4 w = l4[k];
+        v[4 * k] = w.x;
+        v[4 * k + 1] = w.y;
+        v[4 * k + 2] = w.z;
+        v[4 * k + 3] = w.w;
+    }
     int n = cnt[c];
     n = n < cap ? n : cap;
-    int *l = bkt + (size_t)c * cap;
-    if (n <= 16 ? sort_bin_in_registers<16>(l, n)
-                : (cap >= 32 && sort_bin_in_registers<32>(l, n))) {
+    if (n <= 16) {
+        int (&v16)[16] = *reinterpret_cast<int (*)[16]>(v);
+        if (!sort_in_registers<16>(v16, n)) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+            }
+        }
+        return;
+    }
+    if (n <= 32) {
+#pragma unroll
+        for (int k = 4; k < 8; ++k) {
+            const int4 w = l4[k];
+            v[4 * k] = w.x;
+            v[4 * k + 1] = w.y;
+            v[4 * k + 2] = w.z;
+            v[4 * k + 3] = w.w;
+        }
+        if (!sort_in_registers<32>(v, n)) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+            }
+        }
         return;
     }
     for (int k = 1; k < n; ++k) {
@@ -2359,14 +2377,40 @@ __global__ __launch_bounds__(256) void k_copy_state(StateCopy c)
 __global__ __launch_bounds__(256) void k_adj_reduce(const double *__restrict__ partials,
                                                     int nblocks, double *__restrict__ acc)
 {
-    __shared__ double s_red[4];
+    // the seven sums side by side, each in block_sum's order
+    __shared__ double s_red[kAdjParams][4];
+    double v[kAdjParams];
+#pragma unroll
     for (int k = 0; k < kAdjParams; ++k) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-            v += partials[(size_t)k * nblocks + b];
+        v[k] = 0.0;
+    }
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < kAdjParams; ++k) {
+            v[k] += partials[(size_t)k * nblocks + b];
         }
-        v = block_sum(v, s_red);
-        if (threadIdx.x == 0) acc[k] += v;
+    }
+#pragma unroll
+    for (int k = 0; k < kAdjParams; ++k) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            v[k] += __shfl_down(v[k], d);
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kAdjParams; ++k) {
+            s_red[k][w] = v[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kAdjParams) {
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+            t += s_red[threadIdx.x][q];
+        }
+        acc[threadIdx.x] += t;
     }
 }
 
