@@ -57,7 +57,8 @@ constexpr int kColourCellsMax = ((kRegX + 1) / 2) * ((kRegY + 1) / 2);   // 9 x 
 constexpr int kStageMax = 720;     // staged records per batch (64 B each); a multiple
                                    // of 16, the LDS bank swizzle's block
 constexpr int kTileNodes = kTileX * kTileY;
-constexpr int kP2GThreads = 2 * kTileNodes;   // two threads per node
+// k_p2g_grid runs two or three threads per node (ROLES): the node's five
+// sums are split between them.
 constexpr int kG2PThreads = 256;
 constexpr int kBatchTileChunk = 16;   // ensemble P2G tiles per XCD chunk
 // Particle count limit: byte offsets into a particle field fit the 31-bit
@@ -458,17 +459,20 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
 // MUSL2: the second mapping of the MUSL step (process_usl.c:409-440): only
 // the mass and momentum chains, then v = p / m without boundary conditions
 // or the force update.
-template <int CAP, bool BATCH, bool MUSL2 = false>
-__global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
+template <int CAP, bool BATCH, bool MUSL2, int ROLES>
+__global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
     int tiles_x, int ntiles, int chunk, int np, BatchTables bt)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
+    constexpr int kP2GThreads = ROLES * kTileNodes;
+    constexpr int kWaves = kP2GThreads / 64;
     constexpr int kStageSlots = (kStageMax + kP2GThreads - 1) / kP2GThreads;   // per thread
     constexpr int kMaxBatches = 8;   // 45 cells of at most 64 particles: <= 5 batches
-    static_assert(kP2GThreads / 64 == 4, "one wave per colour scans the colour's cells");
+    static_assert(ROLES == 2 || ROLES == 3, "two or three threads per node");
+    static_assert(kWaves >= 4, "one wave per colour scans the colour's cells");
 
     __shared__ int s_cnt[kRegCells];
     // per colour: exclusive scan of the cells' counts (compact list offsets),
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     // per colour: the cell holding list slot 16 b (slots are looked up from
     // there: one step for bins of 16 entries)
     __shared__ unsigned char s_blk[4][(kColourCellsMax * CAP + 15) / 16];
-    __shared__ int s_inv[kP2GThreads / 64];   // a wave saw a bin out of index order
+    __shared__ int s_inv[kWaves];   // a wave saw a bin out of index order
     // a batch's particle indices in bin order (used only to rank the
     // entries of bins that are out of index order)
     __shared__ int s_cmp[kStageSlots * kP2GThreads];
@@ -516,9 +520,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const int ty0 = (tile / tiles_x) * kTileY;
     const int cx0 = tx0 - 1, cy0 = ty0 - 1;   // staged cell region origin
     const int tid = threadIdx.x;
-    // Two threads per node: role 0 folds the m, mx, fx chains, role 1 the
-    // my, fy chains (each chain is one node field, summed in the reference's
-    // order by exactly one thread).
+    // Each chain is one node field, summed in the reference's order by
+    // exactly one thread.  Two roles: role 0 folds the m, mx, fx chains,
+    // role 1 the my, fy chains.  Three roles: role 0 the m, mx, my chains,
+    // role 1 fx, role 2 fy (shorter chains, more waves per tile).
     const int role = __builtin_amdgcn_readfirstlane(tid / kTileNodes);   // uniform per wave
     const int nt = tid % kTileNodes;
 
@@ -549,7 +554,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     auto ncx_of = [](int c) { return (c & 1) ? (kRegX + 1) / 2 : kRegX / 2; };
     auto ncy_of = [](int c) { return (c >> 1) ? (kRegY + 1) / 2 : kRegY / 2; };
 
-    if (any_particles) {
+    if (any_particles && tid < 4 * 64) {
         // Wave w lays out colour w: list offsets (a scan of the counts) and
         // the staging batches.  Offsets never decrease and one cell holds at
         // most CAP <= kStageMax particles, so the cells fitting a batch form
@@ -666,8 +671,11 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     const int i = tx0 + (nt % kTileX);
     const int j = ty0 + (nt / kTileX);
     const bool live = i < g.N && j < g.N;
-    const double bb = role == 0 ? st->bx : st->by;   // body force of this role's chain
-    double am = 0.0, ap = 0.0, af = 0.0;             // role 0: m, mx, fx; role 1: -, my, fy
+    // body force of this role's force chain
+    const double bb = role == ROLES - 2 ? st->bx : st->by;
+    // two roles: role 0 m (am), mx (ap), fx (af); role 1 my (ap), fy (af)
+    // three roles: role 0 m (am), mx (ap), my (ay); roles 1, 2 fx, fy (af)
+    double am = 0.0, ap = 0.0, ay = 0.0, af = 0.0;
 
     if (any_particles) {
     Slots Rn;
@@ -699,7 +707,7 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
         // is exact; adding -0 leaves any value unchanged), and grad_tent's
         // sign folded into the 1 / h factor
         const double ax = dx ? 1.0 : -1.0, cx = dx ? -0.0 : 1.0;
-        const double ay = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
+        const double ay_ = dy ? 1.0 : -1.0, cy = dy ? -0.0 : 1.0;
         const double hx = dx ? g.inv_h : -g.inv_h, hy = dy ? g.inv_h : -g.inv_h;
 
         const int nbat = s_nbat[c];
@@ -715,7 +723,12 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             for (int u = 0; u < kStageSlots; ++u) {
                 dst[u] = tid + u * kP2GThreads;
             }
-            if (__builtin_amdgcn_readfirstlane(s_inv[0] | s_inv[1] | s_inv[2] | s_inv[3])) {
+            int any_inv = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                any_inv |= s_inv[w];
+            }
+            if (__builtin_amdgcn_readfirstlane(any_inv)) {
 #pragma unroll
                 for (int u = 0; u < kStageSlots; ++u) {
                     if (Rc.q[u] >= 0) {
@@ -782,22 +795,26 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
             if (myq >= qb && myq < qe) {
                 const int k0 = s_off[c][myq] - base;
                 const int k1 = s_off[c][myq + 1] - base;
-                // role 0 with no x body force: af starts at +0 and is only
-                // ever a sum, so it is never -0, and adding sf * (0 * m)
-                // (+-0 for a finite mass) leaves it unchanged -- the term
-                // is skipped.  One loop per role (the role is uniform per
-                // wave), so neither carries the other's selects.
-                auto fold0 = [&](auto body_force) {
+                // with no x body force the fx chain drops sf * (0 * m): af
+                // starts at +0 and is only ever a sum, so it is never -0,
+                // and adding +-0 (a finite mass) leaves it unchanged.  One
+                // loop per role (the role is uniform per wave), so none
+                // carries another's selects.
+                const bool fx_role = ROLES == 2 ? role == 0 : role == 1;
+                const bool bf = !(fx_role && g.m_finite && bb == 0.0);
+                auto fold_fx = [&](auto body_force, auto with_m) {
 #pragma unroll 2
                     for (int s0 = k0; s0 < k1; ++s0) {
                         const int s = swz(s0);
                         const double2 xy = s_xy[s], msy = s_msy[s], ps = s_pxs[s];
                         const double X = fma(ax, xy.x, cx);
-                        const double Y = fma(ay, xy.y, cy);
+                        const double Y = fma(ay_, xy.y, cy);
                         const double sf = X * Y;
                         const double mp = msy.x;
-                        am += sf * mp;
-                        ap += sf * ps.x;
+                        if (decltype(with_m)::value) {
+                            am += sf * mp;
+                            ap += sf * ps.x;
+                        }
                         if (!MUSL2) {
                             if (decltype(body_force)::value) {
                                 af += sf * (bb * mp);
@@ -807,26 +824,54 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
                         }
                     }
                 };
-                if (role == 0) {
-                    if (g.m_finite && bb == 0.0) {
-                        fold0(std::false_type{});
-                    } else {
-                        fold0(std::true_type{});
-                    }
-                } else {
+                auto fold_fy = [&](auto with_my) {
 #pragma unroll 2
                     for (int s0 = k0; s0 < k1; ++s0) {
                         const int s = swz(s0);
                         const double2 xy = s_xy[s], msy = s_msy[s], ps = s_pys[s];
                         const double X = fma(ax, xy.x, cx);
-                        const double Y = fma(ay, xy.y, cy);
+                        const double Y = fma(ay_, xy.y, cy);
                         const double sf = X * Y;
-                        ap += sf * ps.x;
+                        if (decltype(with_my)::value) {
+                            ap += sf * ps.x;
+                        }
                         if (!MUSL2) {
                             af += sf * (bb * msy.x);
                             af += (Y * hx) * msy.y;    // -sxy v
                             af += (X * hy) * ps.y;     // -syy v
                         }
+                    }
+                };
+                if (ROLES == 2) {
+                    if (role == 0) {
+                        if (bf) {
+                            fold_fx(std::true_type{}, std::true_type{});
+                        } else {
+                            fold_fx(std::false_type{}, std::true_type{});
+                        }
+                    } else {
+                        fold_fy(std::true_type{});
+                    }
+                } else if (role == 0) {
+#pragma unroll 2
+                    for (int s0 = k0; s0 < k1; ++s0) {
+                        const int s = swz(s0);
+                        const double2 xy = s_xy[s];
+                        const double mp = s_msy[s].x, pxm = s_pxs[s].x, pym = s_pys[s].x;
+                        const double sf = fma(ax, xy.x, cx) * fma(ay_, xy.y, cy);
+                        am += sf * mp;
+                        ap += sf * pxm;
+                        ay += sf * pym;
+                    }
+                } else if (!MUSL2) {
+                    if (role == 1) {
+                        if (bf) {
+                            fold_fx(std::true_type{}, std::false_type{});
+                        } else {
+                            fold_fx(std::false_type{}, std::false_type{});
+                        }
+                    } else {
+                        fold_fy(std::false_type{});
                     }
                 }
             }
@@ -839,19 +884,27 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g_grid(
     }
     }
 
-    // Hand the y chains to the role-0 thread of the node (staging space is
-    // free after the last barrier).
+    // Hand the other roles' chains to the role-0 thread of the node
+    // (staging space is free after the last barrier).
     double2 *s_y = s_xy;
-    if (role == 1) {
-        s_y[nt] = make_double2(ap, af);
+    if (role != 0) {
+        s_y[(role - 1) * kTileNodes + nt] = make_double2(ap, af);
     }
     __syncthreads();
     if (role != 0 || !live) {
         return;
     }
-    double amx = ap, afx = af;
-    const double2 yv = s_y[nt];
-    double amy = yv.x, afy = yv.y;
+    double amx = ap, amy, afx, afy;
+    if (ROLES == 2) {
+        const double2 yv = s_y[nt];
+        afx = af;
+        amy = yv.x;
+        afy = yv.y;
+    } else {
+        amy = ay;
+        afx = s_y[nt].y;
+        afy = s_y[kTileNodes + nt].y;
+    }
     if (MUSL2) {
         NodeRec r;
         r.m = am;
@@ -2451,6 +2504,7 @@ struct mpm_hip_ctx {
     DevState *st;
     int cur;                // bins holding the current step's lists
     int tiles_x, ntiles, p2g_blocks;
+    int p2g_roles;          // threads per node in k_p2g_grid (2 or 3)
     bool have_particles;
     // g_nonlocal
     bool nonlocal;
@@ -2548,27 +2602,37 @@ static int use_device(const mpm_hip_ctx *ctx)
     return MPM_HIP_OK;
 }
 
+template <bool MUSL2, int ROLES>
+static void p2g_kernel_launch_roles(mpm_hip_ctx *ctx)
+{
+    const int cur = ctx->cur, nxt = 1 - cur;
+    const dim3 grid(ctx->p2g_blocks), block(ROLES * kTileNodes);
+    switch (ctx->g.cap) {
+    case 16:
+        k_p2g_grid<16, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+        break;
+    case 32:
+        k_p2g_grid<32, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+        break;
+    default:
+        k_p2g_grid<64, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+        break;
+    }
+}
+
 template <bool MUSL2>
 static void p2g_kernel_launch(mpm_hip_ctx *ctx)
 {
-    const int cur = ctx->cur, nxt = 1 - cur;
-    const dim3 grid(ctx->p2g_blocks), block(kP2GThreads);
-    switch (ctx->g.cap) {
-    case 16:
-        k_p2g_grid<16, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
-            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
-        break;
-    case 32:
-        k_p2g_grid<32, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
-            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
-        break;
-    default:
-        k_p2g_grid<64, false, MUSL2><<<grid, block, 0, ctx->stream>>>(ctx->dP, ctx->cnt[cur],
-            ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st, ctx->tiles_x,
-            ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
-        break;
+    if (ctx->p2g_roles == 3) {
+        p2g_kernel_launch_roles<MUSL2, 3>(ctx);
+    } else {
+        p2g_kernel_launch_roles<MUSL2, 2>(ctx);
     }
 }
 
@@ -2688,20 +2752,20 @@ static BatchTables batch_tables(const mpm_hip_batch *b)
 static int batch_launch_p2g(mpm_hip_batch *b, int cur)
 {
     const int nxt = 1 - cur;
-    const dim3 grid(b->p2g_blocks), block(kP2GThreads);
+    const dim3 grid(b->p2g_blocks), block(2 * kTileNodes);
     const BatchTables t = batch_tables(b);
     const GridParams g0{};
     switch (b->cap) {
     case 16:
-        k_p2g_grid<16, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<16, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     case 32:
-        k_p2g_grid<32, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<32, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     default:
-        k_p2g_grid<64, true><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
+        k_p2g_grid<64, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
             b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
         break;
     }
@@ -2971,6 +3035,10 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     ctx->tiles_x = (cfg->N + kTileX - 1) / kTileX;
     ctx->ntiles = ctx->tiles_x * ((cfg->N + kTileY - 1) / kTileY);
     ctx->p2g_blocks = p2g_grid_blocks(ctx->ntiles, ctx->tiles_x);
+    // three threads per node when every tile can be resident at once (two
+    // 384-thread workgroups per CU): shorter chains per thread for a grid
+    // that is all latency; two otherwise, for three workgroups per CU
+    ctx->p2g_roles = ctx->ntiles <= 2 * prop.multiProcessorCount ? 3 : 2;
 
 #define ALLOC_OR_FAIL(ptr, bytes)                                                   \
     do {                                                                            \
