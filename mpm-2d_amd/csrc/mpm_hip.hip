@@ -34,6 +34,7 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -130,6 +131,22 @@ struct MemberInfo {
     Material mt;
     int node_base, cell_base, part_base, np;
     int tile_base, tiles_x;
+};
+
+// Dispatch order of a single job's P2G tiles, in three ranges of
+// workgroups: the core tiles of the particles (cx0.., cy0.., cw x ch tiles),
+// then the rest of the box around them (bx0.., by0.., bw x bh tiles: every
+// tile whose cell region held a particle when the particles were last
+// uploaded; its rim holds the lightly loaded edge tiles), then the tiles
+// outside the box.  The tiles with work start first instead of queueing
+// behind empty ones, and the heavy tiles are dealt evenly to the CUs before
+// the light ones take the spare workgroup slots.  Any order gives the same
+// node sums (each tile is written by one workgroup), so a stale box only
+// costs time.
+struct TileOrder {
+    int cx0, cy0, cw, ch;
+    int bx0, by0, bw, bh;
+    int blocks_a, blocks_b;   // workgroups of the core and of the rim
 };
 
 // Ensemble tables (device memory); unused by single-job launches.
@@ -456,6 +473,56 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     bin_insert(e >= 0, e, p, cnt, bkt, g.cap, &st->status);
 }
 
+// The v-th tile, in row-major order, of the x0.., y0.., w x h tiles that lie
+// outside the inner rectangle ix0.., iy0.., iw x ih within them.
+__device__ __forceinline__ int rect_rim_tile(int v, int x0, int y0, int w, int h, int ix0,
+                                             int iy0, int iw, int ih, int tiles_x)
+{
+    (void)h;
+    const int n0 = (iy0 - y0) * w;   // rows below the inner rectangle
+    if (v < n0) {
+        return (y0 + v / w) * tiles_x + x0 + v % w;
+    }
+    v -= n0;
+    const int wr = w - iw;           // tiles beside it in each of its rows
+    const int n1 = ih * wr;
+    if (v < n1) {
+        const int c = v % wr;
+        return (iy0 + v / wr) * tiles_x + x0 + (c < ix0 - x0 ? c : c + iw);
+    }
+    v -= n1;
+    return (iy0 + ih + v / w) * tiles_x + x0 + v % w;
+}
+
+// Tile of P2G workgroup b of a single job (TileOrder), -1 for a spare one.
+// Within each range, chunks of consecutive tiles (a row of the core, one
+// tile of the rim, a grid row outside) go to the 8 XCDs in turn: workgroups
+// are dealt to XCDs round-robin, so the tiles of a chunk share an XCD's L2
+// while any patch of the grid that holds particles is spread over all eight.
+__device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const TileOrder &o)
+{
+    auto chunked = [](int b, int chunk) {
+        const int xcd = b & 7, slot = b >> 3;
+        return ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
+    };
+    if (b < o.blocks_a) {
+        const int v = chunked(b, o.cw);
+        return v < o.cw * o.ch ? (o.cy0 + v / o.cw) * tiles_x + o.cx0 + v % o.cw : -1;
+    }
+    b -= o.blocks_a;
+    if (b < o.blocks_b) {
+        const int v = chunked(b, 1);
+        return v < o.bw * o.bh - o.cw * o.ch
+                   ? rect_rim_tile(v, o.bx0, o.by0, o.bw, o.bh, o.cx0, o.cy0, o.cw, o.ch, tiles_x)
+                   : -1;
+    }
+    b -= o.blocks_b;
+    const int v = chunked(b, tiles_x);
+    return v < ntiles - o.bw * o.bh
+               ? rect_rim_tile(v, 0, 0, tiles_x, ntiles / tiles_x, o.bx0, o.by0, o.bw, o.bh, tiles_x)
+               : -1;
+}
+
 // MUSL2: the second mapping of the MUSL step (process_usl.c:409-440): only
 // the mass and momentum chains, then v = p / m without boundary conditions
 // or the force update.
@@ -464,7 +531,7 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
-    int tiles_x, int ntiles, int chunk, int np, BatchTables bt)
+    int tiles_x, int ntiles, int chunk, int np, TileOrder ord, BatchTables bt)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
     constexpr int kP2GThreads = ROLES * kTileNodes;
@@ -495,13 +562,16 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
     __shared__ double2 s_pys[kStageMax];    // vy * m, -syy * v
 
-    // Workgroups are dealt to the 8 XCDs round-robin.  Chunks of `chunk`
-    // consecutive tiles (a tile row of a single grid) go to the XCDs in turn,
-    // so tiles of one chunk share an XCD's L2 while any patch of the grid
-    // that holds particles is spread over all eight XCDs.
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    int tile = ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
-    if (tile >= ntiles) {
+    // An ensemble's tiles: chunks of `chunk` consecutive tiles go to the 8
+    // XCDs in turn (as in p2g_tile), members in order.
+    int tile;
+    if constexpr (BATCH) {
+        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+        tile = ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
+    } else {
+        tile = p2g_tile(blockIdx.x, tiles_x, ntiles, ord);
+    }
+    if (tile < 0 || tile >= ntiles) {
         return;
     }
     if constexpr (BATCH) {
@@ -2504,6 +2574,7 @@ struct mpm_hip_ctx {
     DevState *st;
     int cur;                // bins holding the current step's lists
     int tiles_x, ntiles, p2g_blocks;
+    TileOrder order;        // k_p2g_grid's dispatch order (p2g_blocks follows it)
     int p2g_roles;          // threads per node in k_p2g_grid (2 or 3)
     bool have_particles;
     // g_nonlocal
@@ -2586,6 +2657,84 @@ static int p2g_grid_blocks(int ntiles, int chunk)
     return ((chunks + 7) / 8) * 8 * chunk;
 }
 
+// Dispatch order of k_p2g_grid (TileOrder): core, rim of the box, the rest.
+static void set_tile_order(mpm_hip_ctx *ctx, int cx0, int cy0, int cw, int ch, int bx0, int by0,
+                           int bw, int bh)
+{
+    TileOrder &o = ctx->order;
+    o.cx0 = cx0;
+    o.cy0 = cy0;
+    o.cw = cw;
+    o.ch = ch;
+    o.bx0 = bx0;
+    o.by0 = by0;
+    o.bw = bw;
+    o.bh = bh;
+    const int nc = cw * ch, nb = bw * bh;
+    o.blocks_a = p2g_grid_blocks(nc, cw);
+    o.blocks_b = nb > nc ? p2g_grid_blocks(nb - nc, 1) : 0;
+    ctx->p2g_blocks = o.blocks_a + o.blocks_b
+                      + (nb < ctx->ntiles ? p2g_grid_blocks(ctx->ntiles - nb, ctx->tiles_x) : 0);
+}
+
+// Row order over the whole grid: no hint.
+static void set_row_order(mpm_hip_ctx *ctx)
+{
+    const int ty = ctx->ntiles / ctx->tiles_x;
+    set_tile_order(ctx, 0, 0, ctx->tiles_x, ty, 0, 0, ctx->tiles_x, ty);
+}
+
+// The box of tiles whose cell region (17 x 9 cells from one below and left
+// of the tile's first node) holds a particle on the grid, and its core: the
+// box without an edge column or row that carries less than half the load of
+// the box's heaviest column or row (the one-cell halo of a block of
+// particles).  Row order when no particle is on the grid.
+static void order_tiles_by_particles(mpm_hip_ctx *ctx, const double *x, const double *y)
+{
+    const int tx = ctx->tiles_x, ty = ctx->ntiles / ctx->tiles_x;
+    const GridParams &g = ctx->g;
+    long long *col = new (std::nothrow) long long[tx + ty]();
+    if (!col) {
+        set_row_order(ctx);
+        return;
+    }
+    long long *row = col + tx;
+    int c0 = tx, c1 = -1, r0 = ty, r1 = -1;
+    for (size_t p = 0; p < ctx->np; ++p) {
+        const double fx = std::floor(x[p] / g.h), fy = std::floor(y[p] / g.h);
+        if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)g.Ne && fy < (double)g.Ne)) {
+            continue;
+        }
+        // cell c lies in the regions of tile columns c / 16 and (c + 1) / 16
+        const int c = (int)fx, r = (int)fy;
+        const int ta = c / kTileX, tb = std::min((c + 1) / kTileX, tx - 1);
+        const int ra = r / kTileY, rb = std::min((r + 1) / kTileY, ty - 1);
+        col[ta] += 1;
+        col[tb] += tb != ta;
+        row[ra] += 1;
+        row[rb] += rb != ra;
+        c0 = std::min(c0, ta);
+        c1 = std::max(c1, tb);
+        r0 = std::min(r0, ra);
+        r1 = std::max(r1, rb);
+    }
+    if (c1 < 0) {
+        delete[] col;
+        set_row_order(ctx);
+        return;
+    }
+    long long cmax = 0, rmax = 0;
+    for (int t = c0; t <= c1; ++t) cmax = std::max(cmax, col[t]);
+    for (int t = r0; t <= r1; ++t) rmax = std::max(rmax, row[t]);
+    int k0 = c0, k1 = c1, q0 = r0, q1 = r1;
+    if (k1 > k0 && 2 * col[k0] < cmax) ++k0;
+    if (k1 > k0 && 2 * col[k1] < cmax) --k1;
+    if (q1 > q0 && 2 * row[q0] < rmax) ++q0;
+    if (q1 > q0 && 2 * row[q1] < rmax) --q1;
+    delete[] col;
+    set_tile_order(ctx, k0, q0, k1 - k0 + 1, q1 - q0 + 1, c0, r0, c1 - c0 + 1, r1 - r0 + 1);
+}
+
 static int all_finite(const double *a, size_t n)
 {
     for (size_t k = 0; k < n; ++k) {
@@ -2611,17 +2760,17 @@ static void p2g_kernel_launch_roles(mpm_hip_ctx *ctx)
     case 16:
         k_p2g_grid<16, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     case 32:
         k_p2g_grid<32, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     default:
         k_p2g_grid<64, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     }
 }
@@ -2758,15 +2907,15 @@ static int batch_launch_p2g(mpm_hip_batch *b, int cur)
     switch (b->cap) {
     case 16:
         k_p2g_grid<16, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     case 32:
         k_p2g_grid<32, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     default:
         k_p2g_grid<64, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, t);
+            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -3034,7 +3183,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
 
     ctx->tiles_x = (cfg->N + kTileX - 1) / kTileX;
     ctx->ntiles = ctx->tiles_x * ((cfg->N + kTileY - 1) / kTileY);
-    ctx->p2g_blocks = p2g_grid_blocks(ctx->ntiles, ctx->tiles_x);
+    set_row_order(ctx);
     // three threads per node when every tile can be resident at once (two
     // 384-thread workgroups per CU): shorter chains per thread for a grid
     // that is all latency; two otherwise, for three workgroups per CU
@@ -3185,6 +3334,7 @@ int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
         }
     }
     ctx->g.m_finite = all_finite(m, ctx->np);
+    order_tiles_by_particles(ctx, x, y);
     DevState s0;
     memset(&s0, 0, sizeof s0);
     s0.t = ctx->cfg.t0;
@@ -3212,6 +3362,8 @@ int mpm_hip_set_field(mpm_hip_ctx *ctx, int field, const double *host)
         ctx->g.m_finite = all_finite(host, ctx->np);
     }
     if (field == MPM_F_X || field == MPM_F_Y) {
+        // positions changed: no dispatch hint
+        set_row_order(ctx);
         rc = rebin(ctx, 0);
         if (rc) return rc;
     }
