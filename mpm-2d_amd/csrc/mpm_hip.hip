@@ -147,6 +147,7 @@ struct TileOrder {
     int cx0, cy0, cw, ch;
     int bx0, by0, bw, bh;
     int blocks_a, blocks_b;   // workgroups of the core and of the rim
+    int core_rows;            // core rows per XCD chunk
 };
 
 // Ensemble tables (device memory); unused by single-job launches.
@@ -161,9 +162,6 @@ struct BatchTables {
 // Point operations shared by both kernels.
 // ---------------------------------------------------------------------------
 
-// WHICH_ELEMENT4 (process_usl.c:43-44).  An index at or beyond the element
-// count (an ulp-level rounding case the reference reads out of bounds for)
-// is treated as outside.
 // a / h.  When h is a power of two the quotient is exact, and so is the
 // product with 1 / h (a power of two too): the same value, for one
 // multiplication instead of the division sequence.
@@ -172,6 +170,9 @@ __device__ __forceinline__ double div_h(double a, const GridParams &g)
     return g.h_pow2 ? a * g.inv_h : a / g.h;
 }
 
+// WHICH_ELEMENT4 (process_usl.c:43-44).  An index at or beyond the element
+// count (an ulp-level rounding case the reference reads out of bounds for)
+// is treated as outside.
 __device__ __forceinline__ int which_element(double x, double y, const GridParams &g)
 {
     if (x < 1.0 && x >= 0.0 && y < 1.0 && y >= 0.0) {
@@ -495,10 +496,11 @@ __device__ __forceinline__ int rect_rim_tile(int v, int x0, int y0, int w, int h
 }
 
 // Tile of P2G workgroup b of a single job (TileOrder), -1 for a spare one.
-// Within each range, chunks of consecutive tiles (a row of the core, one
+// Within each range, chunks of consecutive tiles (a band of core rows, one
 // tile of the rim, a grid row outside) go to the 8 XCDs in turn: workgroups
 // are dealt to XCDs round-robin, so the tiles of a chunk share an XCD's L2
-// while any patch of the grid that holds particles is spread over all eight.
+// (a band's tiles read their neighbours' edge cells from it) while any
+// patch of the grid that holds particles is spread over all eight.
 __device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const TileOrder &o)
 {
     auto chunked = [](int b, int chunk) {
@@ -506,7 +508,7 @@ __device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const Ti
         return ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
     };
     if (b < o.blocks_a) {
-        const int v = chunked(b, o.cw);
+        const int v = chunked(b, o.cw * o.core_rows);
         return v < o.cw * o.ch ? (o.cy0 + v / o.cw) * tiles_x + o.cx0 + v % o.cw : -1;
     }
     b -= o.blocks_a;
@@ -529,7 +531,7 @@ __device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const Ti
 template <int CAP, bool BATCH, bool MUSL2, int ROLES>
 __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
-    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes,
+    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
     int tiles_x, int ntiles, int chunk, int np, TileOrder ord, BatchTables bt)
 {
@@ -574,6 +576,10 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     if (tile < 0 || tile >= ntiles) {
         return;
     }
+    // 1: the tile's node records are all +0 (written so by the last launch,
+    // or never written since the array was zeroed)
+    const int zero_before = tile_zero[tile];
+    const int flag_index = tile;
     if constexpr (BATCH) {
         const int mb = bt.tile_member[tile];
         const MemberInfo &mi = bt.mem[mb];
@@ -605,7 +611,8 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
         }
     }
     // Bin counts of the 17 x 9 cells around the tile; a tile whose region
-    // holds no particle skips to writing its (zero) node records.
+    // holds no particle skips to writing its (zero) node records, and writes
+    // nothing when they are zero already.
     int occupied = 0;
     for (int cl = tid; cl < kRegCells; cl += kP2GThreads) {
         const int cc = cx0 + cl % kRegX, rr = cy0 + cl / kRegX;
@@ -618,6 +625,12 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
         occupied |= n > 0;
     }
     const bool any_particles = __syncthreads_or(occupied) != 0;
+    if (!any_particles && zero_before) {
+        return;
+    }
+    if (tid == 0 && zero_before != (int)!any_particles) {
+        tile_zero[flag_index] = !any_particles;
+    }
 
     // Colour c's cells: region column a = 2*ax + ca holds cell column cx0 + a
     // (cx0 is odd, so colour parity pc <-> ca = 1 - pc); likewise rows.
@@ -2568,6 +2581,7 @@ struct mpm_hip_ctx {
     Particles *dP;          // device copy of P: kernels load field pointers as used
     double *pfield[MPM_F_COUNT];
     NodeRec *nodes;
+    uint8_t *tile_zero;     // per P2G tile: its node records are all zero
     uint8_t *mask;
     int *cnt[2];
     int *bkt[2];
@@ -2671,7 +2685,8 @@ static void set_tile_order(mpm_hip_ctx *ctx, int cx0, int cy0, int cw, int ch, i
     o.bw = bw;
     o.bh = bh;
     const int nc = cw * ch, nb = bw * bh;
-    o.blocks_a = p2g_grid_blocks(nc, cw);
+    o.core_rows = (ch + 7) / 8;
+    o.blocks_a = p2g_grid_blocks(nc, cw * o.core_rows);
     o.blocks_b = nb > nc ? p2g_grid_blocks(nb - nc, 1) : 0;
     ctx->p2g_blocks = o.blocks_a + o.blocks_b
                       + (nb < ctx->ntiles ? p2g_grid_blocks(ctx->ntiles - nb, ctx->tiles_x) : 0);
@@ -2759,17 +2774,17 @@ static void p2g_kernel_launch_roles(mpm_hip_ctx *ctx)
     switch (ctx->g.cap) {
     case 16:
         k_p2g_grid<16, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
-            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
             ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     case 32:
         k_p2g_grid<32, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
-            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
             ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     default:
         k_p2g_grid<64, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
-            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->mask, ctx->g, ctx->st,
+            ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
             ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
         break;
     }
@@ -2880,6 +2895,7 @@ struct mpm_hip_batch {
     Particles P;
     Particles *dP;             // device copy of P
     NodeRec *nodes;
+    uint8_t *tile_zero;        // per P2G tile: its node records are all zero
     int *cnt[2], *bkt[2];
     size_t np_total, nn_total, ncells_total;
     int ntiles, p2g_blocks, g2p_blocks;
@@ -2907,15 +2923,15 @@ static int batch_launch_p2g(mpm_hip_batch *b, int cur)
     switch (b->cap) {
     case 16:
         k_p2g_grid<16, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     case 32:
         k_p2g_grid<32, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     default:
         k_p2g_grid<64, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -3261,6 +3277,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
     }
     ALLOC_OR_FAIL(P.active, np * sizeof(int));
     ALLOC_OR_FAIL(ctx->nodes, nn * sizeof(NodeRec));
+    ALLOC_OR_FAIL(ctx->tile_zero, (size_t)ctx->ntiles);
     ALLOC_OR_FAIL(ctx->mask, nn);
     for (int b = 0; b < 2; ++b) {
         ALLOC_OR_FAIL(ctx->cnt[b], (size_t)g.ncells * sizeof(int));
@@ -3276,6 +3293,7 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles, mpm_hip_ctx 
 
     if (hipMemsetAsync(ctx->mask, 0, nn, ctx->stream) != hipSuccess
         || hipMemsetAsync(ctx->nodes, 0, nn * sizeof(NodeRec), ctx->stream) != hipSuccess
+        || hipMemsetAsync(ctx->tile_zero, 1, (size_t)ctx->ntiles, ctx->stream) != hipSuccess
         || hipMemsetAsync(P.active, 0, np * sizeof(int), ctx->stream) != hipSuccess
         || hipMemsetAsync(P.elem, 0xff, np * sizeof(int), ctx->stream) != hipSuccess) {
         mpm_hip_destroy(ctx);
@@ -3610,6 +3628,7 @@ int mpm_hip_destroy(mpm_hip_ctx *ctx)
     if (ctx->P.active) (void)hipFree(ctx->P.active);
     if (ctx->dP) (void)hipFree(ctx->dP);
     if (ctx->nodes) (void)hipFree(ctx->nodes);
+    if (ctx->tile_zero) (void)hipFree(ctx->tile_zero);
     if (ctx->mask) (void)hipFree(ctx->mask);
     for (int b = 0; b < 2; ++b) {
         if (ctx->cnt[b]) (void)hipFree(ctx->cnt[b]);
@@ -3734,6 +3753,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     BALLOC(P.elem, np * sizeof(int));
     BALLOC(P.active, np * sizeof(int));
     BALLOC(b->nodes, b->nn_total * sizeof(NodeRec));
+    BALLOC(b->tile_zero, (size_t)(b->ntiles ? b->ntiles : 1));
     for (int k = 0; k < 2; ++k) {
         BALLOC(b->cnt[k], b->ncells_total * sizeof(int));
         BALLOC(b->bkt[k], b->ncells_total * (size_t)cap * sizeof(int));
@@ -3787,6 +3807,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     if (e == hipSuccess) e = hipMemsetAsync(P.elem, 0xff, np * sizeof(int), b->stream);
     if (e == hipSuccess)
         e = hipMemsetAsync(b->nodes, 0, b->nn_total * sizeof(NodeRec), b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->tile_zero, 1, (size_t)b->ntiles, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     free(tm);
     free(bm);
@@ -4087,7 +4108,7 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
     for (int f = 0; f < MPM_F_EXX; ++f) {
         if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
-    void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->cnt[0], b->cnt[1], b->bkt[0],
+    void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->tile_zero, b->cnt[0], b->cnt[1], b->bkt[0],
                     b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member, b->dP};
     for (void *a : arrs) {
         if (a) (void)hipFree(a);
