@@ -153,7 +153,7 @@ struct TileOrder {
 // Ensemble tables (device memory); unused by single-job launches.
 struct BatchTables {
     const MemberInfo *mem;
-    const int *tile_member;    // P2G tile -> member
+    const int2 *tile_order;    // P2G workgroup -> (tile, member); tile -1: a spare one
     const int *block_member;   // particle workgroup -> member
     DevState *st;              // per member
 };
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
-    int tiles_x, int ntiles, int chunk, int np, TileOrder ord, BatchTables bt)
+    int tiles_x, int ntiles, int np, TileOrder ord, BatchTables bt)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
     constexpr int kP2GThreads = ROLES * kTileNodes;
@@ -564,12 +564,12 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
     __shared__ double2 s_pys[kStageMax];    // vy * m, -syy * v
 
-    // An ensemble's tiles: chunks of `chunk` consecutive tiles go to the 8
-    // XCDs in turn (as in p2g_tile), members in order.
-    int tile;
+    // An ensemble's tiles come from the dispatch table built on the host.
+    int tile, mb = 0;
     if constexpr (BATCH) {
-        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-        tile = ((slot / chunk) * 8 + xcd) * chunk + slot % chunk;
+        const int2 tm = bt.tile_order[blockIdx.x];
+        tile = tm.x;
+        mb = tm.y;
     } else {
         tile = p2g_tile(blockIdx.x, tiles_x, ntiles, ord);
     }
@@ -581,7 +581,6 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     const int zero_before = tile_zero[tile];
     const int flag_index = tile;
     if constexpr (BATCH) {
-        const int mb = bt.tile_member[tile];
         const MemberInfo &mi = bt.mem[mb];
         g = mi.g;
         tile -= mi.tile_base;
@@ -2775,17 +2774,17 @@ static void p2g_kernel_launch_roles(mpm_hip_ctx *ctx)
     case 16:
         k_p2g_grid<16, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, (int)ctx->np, ctx->order, BatchTables{});
         break;
     case 32:
         k_p2g_grid<32, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, (int)ctx->np, ctx->order, BatchTables{});
         break;
     default:
         k_p2g_grid<64, false, MUSL2, ROLES><<<grid, block, 0, ctx->stream>>>(ctx->dP,
             ctx->cnt[cur], ctx->bkt[cur], ctx->cnt[nxt], ctx->nodes, ctx->tile_zero, ctx->mask, ctx->g, ctx->st,
-            ctx->tiles_x, ctx->ntiles, ctx->tiles_x, (int)ctx->np, ctx->order, BatchTables{});
+            ctx->tiles_x, ctx->ntiles, (int)ctx->np, ctx->order, BatchTables{});
         break;
     }
 }
@@ -2889,7 +2888,12 @@ struct mpm_hip_batch {
     mpm_hip_config *cfgs;
     unsigned char *have;       // particles set, per member
     MemberInfo *dmem;
-    int *tile_member, *block_member;
+    int2 *tile_order;          // P2G workgroup -> (tile, member), -1 tile: spare
+    int *block_member;
+    int *htile_member;         // host: tile -> member
+    int *tile_load;            // host: particles in each tile's cell region
+    int2 *horder;              // host copy of tile_order
+    bool order_dirty;          // tile_load changed since tile_order was built
     DevState *st;
     double *pfield[MPM_F_EXX];
     Particles P;
@@ -2908,7 +2912,7 @@ static BatchTables batch_tables(const mpm_hip_batch *b)
 {
     BatchTables t;
     t.mem = b->dmem;
-    t.tile_member = b->tile_member;
+    t.tile_order = b->tile_order;
     t.block_member = b->block_member;
     t.st = b->st;
     return t;
@@ -2923,15 +2927,15 @@ static int batch_launch_p2g(mpm_hip_batch *b, int cur)
     switch (b->cap) {
     case 16:
         k_p2g_grid<16, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
         break;
     case 32:
         k_p2g_grid<32, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
         break;
     default:
         k_p2g_grid<64, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, kBatchTileChunk, (int)b->np_total, TileOrder{}, t);
+            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -3760,7 +3764,7 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     }
     BALLOC(b->dmem, (size_t)num_members * sizeof(MemberInfo));
     BALLOC(b->st, (size_t)num_members * sizeof(DevState));
-    BALLOC(b->tile_member, (size_t)b->ntiles * sizeof(int));
+    BALLOC(b->tile_order, (size_t)b->p2g_blocks * sizeof(int2));
     BALLOC(b->block_member, (size_t)b->g2p_blocks * sizeof(int));
     BALLOC(b->dP, sizeof(Particles));
 #undef BALLOC
@@ -3769,11 +3773,13 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         return MPM_HIP_ERR_DEVICE;
     }
 
-    int *tm = (int *)malloc(sizeof(int) * (size_t)(b->ntiles ? b->ntiles : 1));
+    b->htile_member = new (std::nothrow) int[b->ntiles ? b->ntiles : 1];
+    b->tile_load = new (std::nothrow) int[b->ntiles ? b->ntiles : 1]();
+    b->horder = new (std::nothrow) int2[b->p2g_blocks ? b->p2g_blocks : 1];
+    int *tm = b->htile_member;
     int *bm = (int *)malloc(sizeof(int) * (size_t)b->g2p_blocks);
     DevState *s0 = (DevState *)calloc((size_t)num_members, sizeof(DevState));
-    if (!tm || !bm || !s0) {
-        free(tm);
+    if (!tm || !b->tile_load || !b->horder || !bm || !s0) {
         free(bm);
         free(s0);
         mpm_hip_batch_destroy(b);
@@ -3789,11 +3795,9 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         s0[k].t = cfgs[k].t0;
         s0[k].t0 = cfgs[k].loads_t0;
     }
-    hipError_t e = hipMemcpyAsync(b->tile_member, tm, sizeof(int) * (size_t)b->ntiles,
+    b->order_dirty = true;
+    hipError_t e = hipMemcpyAsync(b->block_member, bm, sizeof(int) * (size_t)b->g2p_blocks,
                                   hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->block_member, bm, sizeof(int) * (size_t)b->g2p_blocks,
-                           hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->dmem, b->hmem, sizeof(MemberInfo) * (size_t)num_members,
                            hipMemcpyHostToDevice, b->stream);
@@ -3809,7 +3813,6 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         e = hipMemsetAsync(b->nodes, 0, b->nn_total * sizeof(NodeRec), b->stream);
     if (e == hipSuccess) e = hipMemsetAsync(b->tile_zero, 1, (size_t)b->ntiles, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    free(tm);
     free(bm);
     free(s0);
     if (e != hipSuccess) {
@@ -3817,6 +3820,80 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
         return MPM_HIP_ERR_DEVICE;
     }
     *out = b;
+    return MPM_HIP_OK;
+}
+
+// The load of member k's P2G tiles (particles in each tile's 17 x 9 cell
+// region: cell c lies in the regions of tile columns c / 16 and (c + 1) / 16,
+// likewise rows) from its host positions.
+static void batch_tile_loads(mpm_hip_batch *b, int k, const double *x, const double *y)
+{
+    const MemberInfo &mi = b->hmem[k];
+    const int tx = mi.tiles_x, ty = (b->cfgs[k].N + kTileY - 1) / kTileY;
+    int *load = b->tile_load + mi.tile_base;
+    std::fill(load, load + tx * ty, 0);
+    for (int p = 0; p < mi.np; ++p) {
+        const double fx = std::floor(x[p] / mi.g.h), fy = std::floor(y[p] / mi.g.h);
+        if (!(fx >= 0.0 && fy >= 0.0 && fx < (double)mi.g.Ne && fy < (double)mi.g.Ne)) {
+            continue;
+        }
+        const int c = (int)fx, r = (int)fy;
+        const int ta = c / kTileX, tb = std::min((c + 1) / kTileX, tx - 1);
+        const int ra = r / kTileY, rb = std::min((r + 1) / kTileY, ty - 1);
+        load[ra * tx + ta] += 1;
+        if (tb != ta) load[ra * tx + tb] += 1;
+        if (rb != ra) {
+            load[rb * tx + ta] += 1;
+            if (tb != ta) load[rb * tx + tb] += 1;
+        }
+    }
+    b->order_dirty = true;
+}
+
+// The ensemble's P2G dispatch table (BatchTables::tile_order): tiles carrying
+// at least half the load of their member's heaviest tile first, then the
+// other tiles that hold particles, then the empty ones, each class in tile
+// order, so the heavy tiles start first and are dealt evenly to the CUs;
+// chunks of kBatchTileChunk consecutive entries go to the 8 XCDs in turn.
+// The order is a hint: every tile is in the table once, and any order gives
+// the same node sums.
+static int batch_build_order(mpm_hip_batch *b)
+{
+    const int nt = b->ntiles;
+    int *seq = new (std::nothrow) int[nt ? nt : 1];
+    int *heavy = new (std::nothrow) int[b->nmem];
+    if (!seq || !heavy) {
+        delete[] seq;
+        delete[] heavy;
+        return MPM_HIP_ERR_ALLOC;
+    }
+    for (int k = 0; k < b->nmem; ++k) heavy[k] = 0;
+    for (int t = 0; t < nt; ++t) {
+        int &h = heavy[b->htile_member[t]];
+        h = std::max(h, b->tile_load[t]);
+    }
+    auto cls = [&](int t) {
+        const int l = b->tile_load[t];
+        return l == 0 ? 2 : (2 * l >= heavy[b->htile_member[t]] ? 0 : 1);
+    };
+    int n = 0;
+    for (int c = 0; c < 3; ++c) {
+        for (int t = 0; t < nt; ++t) {
+            if (cls(t) == c) seq[n++] = t;
+        }
+    }
+    constexpr int C = kBatchTileChunk;
+    for (int blk = 0; blk < b->p2g_blocks; ++blk) {
+        const int xcd = blk & 7, slot = blk >> 3;
+        const int v = ((slot / C) * 8 + xcd) * C + slot % C;
+        b->horder[blk] = v < nt ? make_int2(seq[v], b->htile_member[seq[v]]) : make_int2(-1, -1);
+    }
+    delete[] seq;
+    delete[] heavy;
+    HIP_TRY(hipMemcpyAsync(b->tile_order, b->horder, sizeof(int2) * (size_t)b->p2g_blocks,
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));   // horder is rewritten by the next build
+    b->order_dirty = false;
     return MPM_HIP_OK;
 }
 
@@ -3857,6 +3934,7 @@ int mpm_hip_batch_set_particles(mpm_hip_batch *b, int member, const double *m,
     s0.t = b->cfgs[member].t0;
     s0.t0 = b->cfgs[member].loads_t0;
     HIP_TRY(hipMemcpyAsync(b->st + member, &s0, sizeof s0, hipMemcpyHostToDevice, b->stream));
+    batch_tile_loads(b, member, x, y);
     batch_drop_graph(b);
     int rc = batch_rebin(b, member + 1);
     if (rc) return rc;
@@ -3924,6 +4002,9 @@ static bool batch_ready(const mpm_hip_batch *b)
 static int batch_run(mpm_hip_batch *b, int n)
 {
     int rc = MPM_HIP_OK;
+    if (b->order_dirty && (rc = batch_build_order(b))) {
+        return rc;
+    }
     if (n >= kGraphSteps) {
         if (!b->exec) {
             rc = batch_capture(b);
@@ -4109,12 +4190,15 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
         if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
     void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->tile_zero, b->cnt[0], b->cnt[1], b->bkt[0],
-                    b->bkt[1], b->dmem, b->st, b->tile_member, b->block_member, b->dP};
+                    b->bkt[1], b->dmem, b->st, b->tile_order, b->block_member, b->dP};
     for (void *a : arrs) {
         if (a) (void)hipFree(a);
     }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete[] b->hmem;
+    delete[] b->htile_member;
+    delete[] b->tile_load;
+    delete[] b->horder;
     delete[] b->cfgs;
     delete[] b->have;
     delete b;
