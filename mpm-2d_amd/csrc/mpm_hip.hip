@@ -2483,6 +2483,8 @@ struct StateCopy {
     double *dst[MPM_F_EXX];
     const int *asrc;
     int *adst;
+    const int *esrc;   // element indices too, when set
+    int *edst;
     const DevState *ssrc;
     DevState *sdst;
     int *cnt0, *cnt1;
@@ -2498,6 +2500,9 @@ __global__ __launch_bounds__(256) void k_copy_state(StateCopy c)
             c.dst[f][i] = c.src[f][i];
         }
         c.adst[i] = c.asrc[i];
+        if (c.edst) {
+            c.edst[i] = c.esrc[i];
+        }
     }
     if (c.cnt0 && i < c.ncells) {
         c.cnt0[i] = 0;
@@ -4226,6 +4231,18 @@ struct mpm_hip_adjoint {
     double *seg;               // [K][MPM_F_EXX][np]
     int *seg_active;           // [K][np]
     DevState *seg_st;          // [K]
+    // per step of a segment, kept from its forward recompute for the
+    // reverse step: element indices, node records (with their tiles' zero
+    // flags) and the bins of the step's start state
+    int *seg_elem;             // [K][np]
+    NodeRec *seg_nodes;        // [K][N * N]
+    uint8_t *seg_zero;         // [K][P2G tiles]
+    int *seg_cnt;              // [K][cells]
+    int *seg_bkt;              // [K][cells][cap]
+    // the context's own bins and node records (restored after a reverse sweep)
+    int *own_cnt[2], *own_bkt[2];
+    NodeRec *own_nodes;
+    uint8_t *own_zero;
     AdjState A;
     double *lam_store;         // [MPM_F_EXX][np]
     double *acc;               // [kAdjParams]
@@ -4233,9 +4250,12 @@ struct mpm_hip_adjoint {
     bool have_particles, have_forward;
 };
 
-// Saves the context's particle state to (f, act, st) or loads it back; a
-// load also re-bins the particles (bins cleared in the same launch).
-static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, DevState *st)
+// Saves the context's particle state to (f, act, st) or loads it back.
+// With `elem`, the element indices are saved or loaded too and a load keeps
+// the bins as they are; without, a load re-bins the particles into the
+// context's bins 0 (bins cleared in the same launch).
+static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, DevState *st,
+                          int *elem = nullptr)
 {
     mpm_hip_ctx *ctx = a->ctx;
     StateCopy c;
@@ -4249,8 +4269,13 @@ static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, De
     c.adst = save ? act : ctx->P.active;
     c.ssrc = save ? ctx->st : st;
     c.sdst = save ? st : ctx->st;
+    if (elem) {
+        c.esrc = save ? ctx->P.elem : elem;
+        c.edst = save ? elem : ctx->P.elem;
+    }
+    const bool rebin = !save && !elem;
     c.np = (int)a->np;
-    if (!save) {
+    if (rebin) {
         c.cnt0 = ctx->cnt[0];
         c.cnt1 = ctx->cnt[1];
         c.ncells = ctx->g.ncells;
@@ -4258,7 +4283,7 @@ static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, De
     const int n = c.np > c.ncells ? c.np : c.ncells;
     k_copy_state<<<dim3((unsigned)((n > 0 ? n : 1) + 255) / 256), dim3(256), 0, ctx->stream>>>(c);
     HIP_TRY(hipGetLastError());
-    if (!save) {
+    if (rebin) {
         ctx->cur = 0;
         if (a->np) {
             k_bin_particles<false><<<dim3((unsigned)a->g2p_blocks), dim3(kG2PThreads), 0,
@@ -4270,25 +4295,79 @@ static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, De
     return MPM_HIP_OK;
 }
 
-// One reverse step for the step whose start state is loaded in the context.
-static int adj_reverse_step(mpm_hip_adjoint *a)
+// One reverse step for step k of the segment, whose start state is loaded
+// in the context: its node records and bins are those kept by the
+// segment's forward recompute.
+static int adj_reverse_step(mpm_hip_adjoint *a, int k)
 {
     mpm_hip_ctx *ctx = a->ctx;
-    int rc = launch_p2g(ctx);   // the step's node state
-    if (rc) return rc;
     const int np = (int)a->np;
     const int nn = ctx->g.N * ctx->g.N;
+    const NodeRec *nodes = a->seg_nodes + (size_t)k * nn;
+    int *cnt = a->seg_cnt + (size_t)k * ctx->g.ncells;
+    int *bkt = a->seg_bkt + (size_t)k * ctx->g.ncells * ctx->g.cap;
     k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->dP, ctx->nodes, a->A, ctx->g, ctx->mt, np);
+        ctx->dP, nodes, a->A, ctx->g, ctx->mt, np);
     k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->g.ncells, ctx->g.cap);
+        cnt, bkt, ctx->g.ncells, ctx->g.cap);
     k_adj_nodes<<<dim3((4 * nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        ctx->dP, ctx->cnt[ctx->cur], ctx->bkt[ctx->cur], ctx->nodes, ctx->mask, a->A, ctx->g,
-        ctx->st, np);
+        ctx->dP, cnt, bkt, nodes, ctx->mask, a->A, ctx->g, ctx->st, np);
     k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->dP, a->A, ctx->g, ctx->st, np);
     k_adj_reduce<<<dim3(1), dim3(256), 0, ctx->stream>>>(a->A.partials, a->g2p_blocks, a->acc);
     HIP_TRY(hipGetLastError());
+    return MPM_HIP_OK;
+}
+
+// The reverse sweep, segment by segment from the last: each segment's
+// steps are recomputed from its checkpoint, keeping every step's start
+// state, node records and bins (the context's bin and node pointers are
+// pointed at the segment's arrays meanwhile), then reversed from the last.
+static int adj_reverse_segments(mpm_hip_adjoint *a)
+{
+    mpm_hip_ctx *ctx = a->ctx;
+    const size_t np = a->np ? a->np : 1;
+    const size_t nn = (size_t)ctx->g.N * ctx->g.N;
+    const size_t nc = (size_t)ctx->g.ncells, cap = (size_t)ctx->g.cap;
+    int rc = MPM_HIP_OK;
+    const int nseg = (a->nsteps + a->K - 1) / a->K;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int c0 = sg * a->K;
+        const int len = (a->nsteps - c0) < a->K ? (a->nsteps - c0) : a->K;
+        // the checkpoint, binned into step 0's bins
+        ctx->cnt[0] = a->seg_cnt;
+        ctx->bkt[0] = a->seg_bkt;
+        ctx->cnt[1] = a->own_cnt[1];
+        ctx->bkt[1] = a->own_bkt[1];
+        rc = adj_copy_state(a, false, a->cp + (size_t)sg * MPM_F_EXX * np,
+                            a->cp_active + (size_t)sg * np, a->cp_st + sg);
+        if (rc) return rc;
+        for (int k = 0; k < len; ++k) {
+            rc = adj_copy_state(a, true, a->seg + (size_t)k * MPM_F_EXX * np,
+                                a->seg_active + (size_t)k * np, a->seg_st + k,
+                                a->seg_elem + (size_t)k * np);
+            if (rc) return rc;
+            // step k reads bins k and fills bins k + 1 (the context's own
+            // for the last step, whose move is not needed)
+            const int cur = ctx->cur, nxt = 1 - cur;
+            ctx->cnt[cur] = a->seg_cnt + (size_t)k * nc;
+            ctx->bkt[cur] = a->seg_bkt + (size_t)k * nc * cap;
+            ctx->cnt[nxt] = k + 1 < len ? a->seg_cnt + (size_t)(k + 1) * nc : a->own_cnt[1];
+            ctx->bkt[nxt] = k + 1 < len ? a->seg_bkt + (size_t)(k + 1) * nc * cap : a->own_bkt[1];
+            ctx->nodes = a->seg_nodes + (size_t)k * nn;
+            ctx->tile_zero = a->seg_zero + (size_t)k * ctx->ntiles;
+            if ((rc = launch_p2g(ctx))) return rc;
+            if (k + 1 < len && (rc = launch_g2p(ctx))) return rc;
+        }
+        for (int k = len - 1; k >= 0; --k) {
+            rc = adj_copy_state(a, false, a->seg + (size_t)k * MPM_F_EXX * np,
+                                a->seg_active + (size_t)k * np, a->seg_st + k,
+                                a->seg_elem + (size_t)k * np);
+            if (rc) return rc;
+            rc = adj_reverse_step(a, k);
+            if (rc) return rc;
+        }
+    }
     return MPM_HIP_OK;
 }
 
@@ -4313,6 +4392,12 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
     }
     a->K = checkpoint_every;
     a->np = num_particles;
+    for (int k = 0; k < 2; ++k) {
+        a->own_cnt[k] = a->ctx->cnt[k];
+        a->own_bkt[k] = a->ctx->bkt[k];
+    }
+    a->own_nodes = a->ctx->nodes;
+    a->own_zero = a->ctx->tile_zero;
     const size_t np = num_particles ? num_particles : 1;
     const size_t nn = (size_t)cfg->N * cfg->N;
     a->g2p_blocks = (int)((np + kG2PThreads - 1) / kG2PThreads);
@@ -4329,6 +4414,11 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
     AALLOC(a->seg, np * sizeof(double) * MPM_F_EXX * (size_t)a->K);
     AALLOC(a->seg_active, np * sizeof(int) * (size_t)a->K);
     AALLOC(a->seg_st, sizeof(DevState) * (size_t)a->K);
+    AALLOC(a->seg_elem, np * sizeof(int) * (size_t)a->K);
+    AALLOC(a->seg_nodes, nn * sizeof(NodeRec) * (size_t)a->K);
+    AALLOC(a->seg_zero, (size_t)a->ctx->ntiles * a->K);
+    AALLOC(a->seg_cnt, (size_t)a->ctx->g.ncells * sizeof(int) * a->K);
+    AALLOC(a->seg_bkt, (size_t)a->ctx->g.ncells * a->ctx->g.cap * sizeof(int) * a->K);
     AALLOC(a->lam_store, np * sizeof(double) * MPM_F_EXX);
     for (int f = 0; f < MPM_F_EXX; ++f) a->A.lam[f] = a->lam_store + (size_t)f * np;
     AALLOC(a->A.coef, np * sizeof(double) * kAdjCoef);
@@ -4336,6 +4426,12 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
     AALLOC(a->A.partials, sizeof(double) * kAdjParams * (size_t)a->g2p_blocks);
     AALLOC(a->acc, sizeof(double) * kAdjParams);
 #undef AALLOC
+    // no segment node records are known to be zero yet
+    if (hipMemsetAsync(a->seg_zero, 0, (size_t)a->ctx->ntiles * a->K, a->ctx->stream)
+        != hipSuccess) {
+        mpm_hip_adjoint_destroy(a);
+        return MPM_HIP_ERR_DEVICE;
+    }
     *out = a;
     return MPM_HIP_OK;
 }
@@ -4440,29 +4536,18 @@ int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
         }
     }
     HIP_TRY(hipMemsetAsync(a->acc, 0, sizeof(double) * kAdjParams, ctx->stream));
-    const int nseg = (a->nsteps + a->K - 1) / a->K;
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int c0 = sg * a->K;
-        const int len = (a->nsteps - c0) < a->K ? (a->nsteps - c0) : a->K;
-        rc = adj_copy_state(a, false, a->cp + (size_t)sg * MPM_F_EXX * np,
-                            a->cp_active + (size_t)sg * np, a->cp_st + sg);
-        if (rc) return rc;
-        for (int k = 0; k < len; ++k) {
-            rc = adj_copy_state(a, true, a->seg + (size_t)k * MPM_F_EXX * np,
-                                a->seg_active + (size_t)k * np, a->seg_st + k);
-            if (rc) return rc;
-            if (k + 1 < len) {
-                if ((rc = launch_p2g(ctx)) || (rc = launch_g2p(ctx))) return rc;
-            }
-        }
-        for (int k = len - 1; k >= 0; --k) {
-            rc = adj_copy_state(a, false, a->seg + (size_t)k * MPM_F_EXX * np,
-                                a->seg_active + (size_t)k * np, a->seg_st + k);
-            if (rc) return rc;
-            rc = adj_reverse_step(a);
-            if (rc) return rc;
-        }
-    }
+    rc = adj_reverse_segments(a);
+    // the context's own bins and node records again, its bins rebuilt for
+    // the state it holds (the first step's start)
+    ctx->cnt[0] = a->own_cnt[0];
+    ctx->cnt[1] = a->own_cnt[1];
+    ctx->bkt[0] = a->own_bkt[0];
+    ctx->bkt[1] = a->own_bkt[1];
+    ctx->nodes = a->own_nodes;
+    ctx->tile_zero = a->own_zero;
+    if (rc) return rc;
+    rc = rebin(ctx, 0);
+    if (rc) return rc;
     double acc[kAdjParams];
     HIP_TRY(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, ctx->stream));
     if (grad_init && bytes) {
@@ -4498,7 +4583,8 @@ int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a)
         (void)hipStreamSynchronize(a->ctx->stream);
     }
     void *arrs[] = {a->init, a->init_active, a->init_st, a->cp, a->cp_active, a->cp_st,
-                    a->seg, a->seg_active, a->seg_st, a->lam_store, a->A.coef, a->A.nadj,
+                    a->seg, a->seg_active, a->seg_st, a->seg_elem, a->seg_nodes, a->seg_zero,
+                    a->seg_cnt, a->seg_bkt, a->lam_store, a->A.coef, a->A.nadj,
                     a->A.partials, a->acc};
     for (void *p : arrs) {
         if (p) (void)hipFree(p);
