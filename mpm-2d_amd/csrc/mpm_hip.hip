@@ -2000,6 +2000,8 @@ struct AdjState {
     double *lam[MPM_F_EXX];     // particle adjoint, updated in place
     double *coef;               // [kAdjCoef][np]
     double *nadj;               // [5][nn]: m~, px~, py~, fx~, fy~
+    double *cellsum;            // [6][4][cells]: per cell and corner, the sums of its
+                                // particles' terms (Fx, Fy, Ux, Uy, Vx, Vy) in bin order
     double *partials;           // [kAdjParams][blocks]
 };
 
@@ -2322,76 +2324,114 @@ __global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_adj_nodes(
+// Per cell, one thread: the terms of the cell's particles for each of its
+// four corner nodes (shape value and gradients times the particle's
+// coefficients), summed in bin order, i.e. ascending particle index.  Each
+// particle is loaded and its shape functions are evaluated once for the
+// four nodes it touches.
+__global__ __launch_bounds__(256) void k_adj_cells(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt, const int *__restrict__ bkt,
-    const NodeRec *__restrict__ nodes, const uint8_t *__restrict__ mask, AdjState A,
-    GridParams g, const DevState *__restrict__ st, int np)
+    AdjState A, GridParams g, int np)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
-    // Four lanes per node, one per colour: lane c gathers the node's cell of
-    // colour c; the four partial sums are then added in colour order.
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nidx = t >> 2, c = t & 3;
-    const int nn = g.N * g.N;
-    const bool in = nidx < nn;
-    const int i = nidx % g.N, j = nidx / g.N;
-    double Fx = 0, Fy = 0, Ux = 0, Uy = 0, Vx = 0, Vy = 0;
-    const int pc = c & 1, pr = c >> 1;
-    const int cc = i - ((i ^ pc) & 1);
-    const int rr = j - ((j ^ pr) & 1);
-    if (in && cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne) {
-        const int dx = i - cc, dy = j - rr;
-        const int q = dy ? (dx ? 2 : 3) : (dx ? 1 : 0);
-        const int cell = rr * g.Ne + cc;
-        int n = cnt[cell];
-        n = n < g.cap ? n : g.cap;
-        const int *bin = bkt + (size_t)cell * g.cap;
-        // particles in chunks of 4: the chunk's loads are all in flight
-        // together; the sums still run in bin order
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            int pk[4];
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.ncells) {
+        return;
+    }
+    const int cc = cell % g.Ne, rr = cell / g.Ne;
+    double F[6][4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                pk[u] = bin[k0 + u < n ? k0 + u : k0];
+    for (int f = 0; f < 6; ++f) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            F[f][q] = 0.0;
+        }
+    }
+    int n = cnt[cell];
+    n = n < g.cap ? n : g.cap;
+    const int *bin = bkt + (size_t)cell * g.cap;
+    // particles in chunks of 4: the chunk's loads are all in flight
+    // together; the sums still run in bin order
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        int pk[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pk[u] = bin[k0 + u < n ? k0 + u : k0];
+        }
+        double px[4], py[4], cf[4][8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            px[u] = P.x[pk[u]];
+            py[u] = P.y[pk[u]];
+#pragma unroll
+            for (int f = 0; f < 8; ++f) {
+                cf[u][f] = A.coef[(size_t)f * np + pk[u]];
             }
-            double px[4], py[4], cf[4][8];
+        }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                px[u] = P.x[pk[u]];
-                py[u] = P.y[pk[u]];
+        for (int u = 0; u < 4; ++u) {
+            if (k0 + u < n) {
+                Shape sh;
+                shape_functions(px[u], py[u], cc, rr, g, sh);
 #pragma unroll
-                for (int f = 0; f < 8; ++f) {
-                    cf[u][f] = A.coef[(size_t)f * np + pk[u]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (k0 + u < n) {
-                    Shape sh;
-                    shape_functions(px[u], py[u], cc, rr, g, sh);
+                for (int q = 0; q < 4; ++q) {
                     const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
-                    Fx += S * cf[u][0];
-                    Fy += S * cf[u][1];
-                    Ux += S * cf[u][2];
-                    Uy += S * cf[u][3];
-                    Vx += b1 * cf[u][4] + b2 * cf[u][5];
-                    Vy += b2 * cf[u][6] + b1 * cf[u][7];
+                    F[0][q] += S * cf[u][0];
+                    F[1][q] += S * cf[u][1];
+                    F[2][q] += S * cf[u][2];
+                    F[3][q] += S * cf[u][3];
+                    F[4][q] += b1 * cf[u][4] + b2 * cf[u][5];
+                    F[5][q] += b2 * cf[u][6] + b1 * cf[u][7];
                 }
             }
         }
     }
-    auto colour_sum = [&](double v) {
-        const int l0 = __lane_id() & ~3;
-        const double v1 = __shfl(v, l0 + 1), v2 = __shfl(v, l0 + 2), v3 = __shfl(v, l0 + 3);
-        return ((v + v1) + v2) + v3;
-    };
-    Fx = colour_sum(Fx);
-    Fy = colour_sum(Fy);
-    Ux = colour_sum(Ux);
-    Uy = colour_sum(Uy);
-    Vx = colour_sum(Vx);
-    Vy = colour_sum(Vy);
-    if (!in || c != 0) return;
+    const size_t nc = (size_t)g.ncells;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            A.cellsum[(f * 4 + q) * nc + cell] = F[f][q];
+        }
+    }
+}
+
+// Per node: the reverse of the grid update and of the boundary zeroing,
+// from the sums of the node's cells (k_adj_cells) added in colour order.
+__global__ __launch_bounds__(256) void k_adj_nodes(
+    const NodeRec *__restrict__ nodes, const uint8_t *__restrict__ mask, AdjState A,
+    GridParams g, const DevState *__restrict__ st)
+{
+    const int nidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nn = g.N * g.N;
+    if (nidx >= nn) {
+        return;
+    }
+    const int i = nidx % g.N, j = nidx / g.N;
+    const size_t nc = (size_t)g.ncells;
+    // the sums of colour c's cell (zero where the node has none), in order
+    double v[4][6];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int pc = c & 1, pr = c >> 1;
+        const int cc = i - ((i ^ pc) & 1);
+        const int rr = j - ((j ^ pr) & 1);
+        const bool has = cc >= 0 && rr >= 0 && cc < g.Ne && rr < g.Ne;
+        const int dx = i - cc, dy = j - rr;
+        const int q = dy ? (dx ? 2 : 3) : (dx ? 1 : 0);
+        const int cell = has ? rr * g.Ne + cc : 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            v[c][f] = has ? A.cellsum[(f * 4 + q) * nc + cell] : 0.0;
+        }
+    }
+    double Fsum[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        Fsum[f] = ((v[0][f] + v[1][f]) + v[2][f]) + v[3][f];
+    }
+    const double Fx = Fsum[0], Fy = Fsum[1], Ux = Fsum[2], Uy = Fsum[3];
+    const double Vx = Fsum[4], Vy = Fsum[5];
     const NodeRec nd = nodes[nidx];
     const double m = nd.m;
     double mb = 0.0, pxb = 0.0, pyb = 0.0, fxb = 0.0, fyb = 0.0;
@@ -4310,8 +4350,10 @@ static int adj_reverse_step(mpm_hip_adjoint *a, int k)
         ctx->dP, nodes, a->A, ctx->g, ctx->mt, np);
     k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         cnt, bkt, ctx->g.ncells, ctx->g.cap);
-    k_adj_nodes<<<dim3((4 * nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        ctx->dP, cnt, bkt, nodes, ctx->mask, a->A, ctx->g, ctx->st, np);
+    k_adj_cells<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        ctx->dP, cnt, bkt, a->A, ctx->g, np);
+    k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        nodes, ctx->mask, a->A, ctx->g, ctx->st);
     k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
         ctx->dP, a->A, ctx->g, ctx->st, np);
     k_adj_reduce<<<dim3(1), dim3(256), 0, ctx->stream>>>(a->A.partials, a->g2p_blocks, a->acc);
@@ -4377,7 +4419,6 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out)
 {
     if (!cfg || !out || checkpoint_every <= 0 || cfg->material_kind != MPM_MAT_G_LOCAL_MU2
-        || cfg->N > 23170   // four gather lanes per node stay within int range
         || cfg->scheme != MPM_SCHEME_USL
         || (cfg->flags & MPM_HIP_HOST_MATERIAL)) {
         return MPM_HIP_ERR_INVALID;
@@ -4423,6 +4464,7 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
     for (int f = 0; f < MPM_F_EXX; ++f) a->A.lam[f] = a->lam_store + (size_t)f * np;
     AALLOC(a->A.coef, np * sizeof(double) * kAdjCoef);
     AALLOC(a->A.nadj, nn * sizeof(double) * 5);
+    AALLOC(a->A.cellsum, (size_t)a->ctx->g.ncells * sizeof(double) * 24);
     AALLOC(a->A.partials, sizeof(double) * kAdjParams * (size_t)a->g2p_blocks);
     AALLOC(a->acc, sizeof(double) * kAdjParams);
 #undef AALLOC
@@ -4584,7 +4626,7 @@ int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a)
     }
     void *arrs[] = {a->init, a->init_active, a->init_st, a->cp, a->cp_active, a->cp_st,
                     a->seg, a->seg_active, a->seg_st, a->seg_elem, a->seg_nodes, a->seg_zero,
-                    a->seg_cnt, a->seg_bkt, a->lam_store, a->A.coef, a->A.nadj,
+                    a->seg_cnt, a->seg_bkt, a->lam_store, a->A.coef, a->A.nadj, a->A.cellsum,
                     a->A.partials, a->acc};
     for (void *p : arrs) {
         if (p) (void)hipFree(p);
