@@ -2651,10 +2651,9 @@ struct mpm_hip_ctx {
         }                                                                          \
     } while (0)
 
-static void bind_fields(mpm_hip_ctx *ctx)
+// Points a particle-array table at field arrays f[MPM_F_*].
+static void bind_fields_to(Particles &P, double *const *f)
 {
-    Particles &P = ctx->P;
-    double **f = ctx->pfield;
     P.x = f[MPM_F_X];
     P.y = f[MPM_F_Y];
     P.vx = f[MPM_F_VX];
@@ -2680,6 +2679,11 @@ static void bind_fields(mpm_hip_ctx *ctx)
     P.gf = f[MPM_F_GF];
     P.gf_bulk = f[MPM_F_GF_BULK];
     P.xisq_inv = f[MPM_F_XISQ_INV];
+}
+
+static void bind_fields(mpm_hip_ctx *ctx)
+{
+    bind_fields_to(ctx->P, ctx->pfield);
 }
 
 // Refreshes the device copy of a particle-array table (after an allocation
@@ -4279,6 +4283,8 @@ struct mpm_hip_adjoint {
     uint8_t *seg_zero;         // [K][P2G tiles]
     int *seg_cnt;              // [K][cells]
     int *seg_bkt;              // [K][cells][cap]
+    Particles *seg_P;          // [K] device tables of the kept start states (the
+                               // reverse kernels read them in place)
     // the context's own bins and node records (restored after a reverse sweep)
     int *own_cnt[2], *own_bkt[2];
     NodeRec *own_nodes;
@@ -4335,27 +4341,28 @@ static int adj_copy_state(mpm_hip_adjoint *a, bool save, double *f, int *act, De
     return MPM_HIP_OK;
 }
 
-// One reverse step for step k of the segment, whose start state is loaded
-// in the context: its node records and bins are those kept by the
-// segment's forward recompute.
+// One reverse step for step k of the segment: its start state, node records
+// and bins are those kept by the segment's forward recompute.
 static int adj_reverse_step(mpm_hip_adjoint *a, int k)
 {
     mpm_hip_ctx *ctx = a->ctx;
+    const Particles *Pk = a->seg_P + k;
+    const DevState *stk = a->seg_st + k;
     const int np = (int)a->np;
     const int nn = ctx->g.N * ctx->g.N;
     const NodeRec *nodes = a->seg_nodes + (size_t)k * nn;
     int *cnt = a->seg_cnt + (size_t)k * ctx->g.ncells;
     int *bkt = a->seg_bkt + (size_t)k * ctx->g.ncells * ctx->g.cap;
     k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->dP, nodes, a->A, ctx->g, ctx->mt, np);
+        Pk, nodes, a->A, ctx->g, ctx->mt, np);
     k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         cnt, bkt, ctx->g.ncells, ctx->g.cap);
     k_adj_cells<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        ctx->dP, cnt, bkt, a->A, ctx->g, np);
+        Pk, cnt, bkt, a->A, ctx->g, np);
     k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        nodes, ctx->mask, a->A, ctx->g, ctx->st);
+        nodes, ctx->mask, a->A, ctx->g, stk);
     k_adj_p2g<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->dP, a->A, ctx->g, ctx->st, np);
+        Pk, a->A, ctx->g, stk, np);
     k_adj_reduce<<<dim3(1), dim3(256), 0, ctx->stream>>>(a->A.partials, a->g2p_blocks, a->acc);
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
@@ -4402,10 +4409,6 @@ static int adj_reverse_segments(mpm_hip_adjoint *a)
             if (k + 1 < len && (rc = launch_g2p(ctx))) return rc;
         }
         for (int k = len - 1; k >= 0; --k) {
-            rc = adj_copy_state(a, false, a->seg + (size_t)k * MPM_F_EXX * np,
-                                a->seg_active + (size_t)k * np, a->seg_st + k,
-                                a->seg_elem + (size_t)k * np);
-            if (rc) return rc;
             rc = adj_reverse_step(a, k);
             if (rc) return rc;
         }
@@ -4465,6 +4468,7 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
     AALLOC(a->A.coef, np * sizeof(double) * kAdjCoef);
     AALLOC(a->A.nadj, nn * sizeof(double) * 5);
     AALLOC(a->A.cellsum, (size_t)a->ctx->g.ncells * sizeof(double) * 24);
+    AALLOC(a->seg_P, sizeof(Particles) * (size_t)a->K);
     AALLOC(a->A.partials, sizeof(double) * kAdjParams * (size_t)a->g2p_blocks);
     AALLOC(a->acc, sizeof(double) * kAdjParams);
 #undef AALLOC
@@ -4473,6 +4477,21 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
         != hipSuccess) {
         mpm_hip_adjoint_destroy(a);
         return MPM_HIP_ERR_DEVICE;
+    }
+    // tables of the kept states: the saved fields, activity and elements
+    for (int k = 0; k < a->K; ++k) {
+        Particles Pk = a->ctx->P;
+        double *f[MPM_F_COUNT];
+        for (int q = 0; q < MPM_F_COUNT; ++q) {
+            f[q] = q < MPM_F_EXX ? a->seg + ((size_t)k * MPM_F_EXX + q) * np : a->ctx->pfield[q];
+        }
+        bind_fields_to(Pk, f);
+        Pk.active = a->seg_active + (size_t)k * np;
+        Pk.elem = a->seg_elem + (size_t)k * np;
+        if (upload_particles(a->seg_P + k, Pk) != MPM_HIP_OK) {
+            mpm_hip_adjoint_destroy(a);
+            return MPM_HIP_ERR_DEVICE;
+        }
     }
     *out = a;
     return MPM_HIP_OK;
@@ -4579,8 +4598,7 @@ int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
     }
     HIP_TRY(hipMemsetAsync(a->acc, 0, sizeof(double) * kAdjParams, ctx->stream));
     rc = adj_reverse_segments(a);
-    // the context's own bins and node records again, its bins rebuilt for
-    // the state it holds (the first step's start)
+    // the context's own bins and node records again, and the initial state
     ctx->cnt[0] = a->own_cnt[0];
     ctx->cnt[1] = a->own_cnt[1];
     ctx->bkt[0] = a->own_bkt[0];
@@ -4588,7 +4606,7 @@ int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
     ctx->nodes = a->own_nodes;
     ctx->tile_zero = a->own_zero;
     if (rc) return rc;
-    rc = rebin(ctx, 0);
+    rc = adj_copy_state(a, false, a->init, a->init_active, a->init_st);
     if (rc) return rc;
     double acc[kAdjParams];
     HIP_TRY(hipMemcpyAsync(acc, a->acc, sizeof acc, hipMemcpyDeviceToHost, ctx->stream));
@@ -4626,7 +4644,8 @@ int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a)
     }
     void *arrs[] = {a->init, a->init_active, a->init_st, a->cp, a->cp_active, a->cp_st,
                     a->seg, a->seg_active, a->seg_st, a->seg_elem, a->seg_nodes, a->seg_zero,
-                    a->seg_cnt, a->seg_bkt, a->lam_store, a->A.coef, a->A.nadj, a->A.cellsum,
+                    a->seg_cnt, a->seg_bkt, a->seg_P, a->lam_store, a->A.coef, a->A.nadj,
+                    a->A.cellsum,
                     a->A.partials, a->acc};
     for (void *p : arrs) {
         if (p) (void)hipFree(p);
