@@ -2264,81 +2264,75 @@ __device__ __forceinline__ bool sort_in_registers(int (&v)[R], int n)
     return false;
 }
 
-// Sorts every cell's bin by particle index (one thread per cell; the
-// adjoint's gather order).  Bins are filled in lane order and are mostly
-// sorted already; bins of up to 32 entries are checked and sorted in
-// registers (the first 16 entries are loaded with the count).
-__global__ __launch_bounds__(256) void k_sort_bins(const int *__restrict__ cnt, int *__restrict__ bkt,
-                                                   int ncells, int cap)
+// Per cell, one thread: the terms of the cell's particles for each of its
+// four corner nodes (shape value and gradients times the particle's
+// coefficients), summed in ascending particle index.  Each particle is
+// loaded and its shape functions are evaluated once for the four nodes it
+// touches.  Bins are filled in lane order and are mostly sorted already;
+// bins of up to 32 entries are checked and sorted in registers (the first
+// 16 entries are loaded with the count) and read back from LDS, longer ones
+// are sorted in place.
+__global__ __launch_bounds__(256) void k_adj_cells(
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt, int *__restrict__ bkt,
+    AdjState A, GridParams g, int np)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int *l = bkt + (size_t)c * cap;
-    int4 *l4 = reinterpret_cast<int4 *>(l);
-    int v[32];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {   // every capacity holds 16 entries
-        const int4 w = l4[k];
-        v[4 * k] = w.x;
-        v[4 * k + 1] = w.y;
-        v[4 * k + 2] = w.z;
-        v[4 * k + 3] = w.w;
-    }
-    int n = cnt[c];
-    n = n < cap ? n : cap;
-    if (n <= 16) {
-        int (&v16)[16] = *reinterpret_cast<int (*)[16]>(v);
-        if (!sort_in_registers<16>(v16, n)) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
-            }
-        }
+    const Particles &P = *Pd;   // field pointers, loaded as used
+    __shared__ int s_bin[256 * 33];   // per thread, stride 33 (no bank conflicts)
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.ncells) {
         return;
     }
-    if (n <= 32) {
+    const int cc = cell % g.Ne, rr = cell / g.Ne;
+    int *l = bkt + (size_t)cell * g.cap;
+    int *sb = s_bin + threadIdx.x * 33;
+    int n;
+    {
+        const int4 *l4 = reinterpret_cast<const int4 *>(l);
+        int v[32];
 #pragma unroll
-        for (int k = 4; k < 8; ++k) {
+        for (int k = 0; k < 4; ++k) {   // every capacity holds 16 entries
             const int4 w = l4[k];
             v[4 * k] = w.x;
             v[4 * k + 1] = w.y;
             v[4 * k + 2] = w.z;
             v[4 * k + 3] = w.w;
         }
-        if (!sort_in_registers<32>(v, n)) {
+        n = cnt[cell];
+        n = n < g.cap ? n : g.cap;
+        if (n <= 16) {
+            int (&v16)[16] = *reinterpret_cast<int (*)[16]>(v);
+            sort_in_registers<16>(v16, n);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                l4[k] = make_int4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+            for (int k = 0; k < 16; ++k) {
+                sb[k] = v[k];
+            }
+        } else if (n <= 32) {
+#pragma unroll
+            for (int k = 4; k < 8; ++k) {
+                const int4 w = l4[k];
+                v[4 * k] = w.x;
+                v[4 * k + 1] = w.y;
+                v[4 * k + 2] = w.z;
+                v[4 * k + 3] = w.w;
+            }
+            sort_in_registers<32>(v, n);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                sb[k] = v[k];
+            }
+        } else {
+            for (int k = 1; k < n; ++k) {
+                const int key = l[k];
+                int r = k - 1;
+                while (r >= 0 && l[r] > key) {
+                    l[r + 1] = l[r];
+                    --r;
+                }
+                l[r + 1] = key;
             }
         }
-        return;
     }
-    for (int k = 1; k < n; ++k) {
-        const int key = l[k];
-        int r = k - 1;
-        while (r >= 0 && l[r] > key) {
-            l[r + 1] = l[r];
-            --r;
-        }
-        l[r + 1] = key;
-    }
-}
-
-// Per cell, one thread: the terms of the cell's particles for each of its
-// four corner nodes (shape value and gradients times the particle's
-// coefficients), summed in bin order, i.e. ascending particle index.  Each
-// particle is loaded and its shape functions are evaluated once for the
-// four nodes it touches.
-__global__ __launch_bounds__(256) void k_adj_cells(
-    const Particles *__restrict__ Pd, const int *__restrict__ cnt, const int *__restrict__ bkt,
-    AdjState A, GridParams g, int np)
-{
-    const Particles &P = *Pd;   // field pointers, loaded as used
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.ncells) {
-        return;
-    }
-    const int cc = cell % g.Ne, rr = cell / g.Ne;
+    const int *bin = n <= 32 ? sb : l;
     double F[6][4];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
@@ -2347,11 +2341,8 @@ __global__ __launch_bounds__(256) void k_adj_cells(
             F[f][q] = 0.0;
         }
     }
-    int n = cnt[cell];
-    n = n < g.cap ? n : g.cap;
-    const int *bin = bkt + (size_t)cell * g.cap;
     // particles in chunks of 4: the chunk's loads are all in flight
-    // together; the sums still run in bin order
+    // together; the sums still run in index order
     for (int k0 = 0; k0 < n; k0 += 4) {
         int pk[4];
 #pragma unroll
@@ -4355,8 +4346,6 @@ static int adj_reverse_step(mpm_hip_adjoint *a, int k)
     int *bkt = a->seg_bkt + (size_t)k * ctx->g.ncells * ctx->g.cap;
     k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
         Pk, nodes, a->A, ctx->g, ctx->mt, np);
-    k_sort_bins<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        cnt, bkt, ctx->g.ncells, ctx->g.cap);
     k_adj_cells<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         Pk, cnt, bkt, a->A, ctx->g, np);
     k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
