@@ -298,7 +298,8 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b);
  * ------------------------------------------------------------------- */
 typedef struct mpm_hip_adjoint mpm_hip_adjoint;
 
-/* g_local_mu2 only (no MPM_HIP_HOST_MATERIAL, no g_nonlocal) and USL. */
+/* g_local_mu2 only (no MPM_HIP_HOST_MATERIAL, no g_nonlocal), USL, and at
+ * most 23170 nodes per side. */
 int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out);
 /* The initial state, as mpm_hip_set_particles. */

@@ -2264,29 +2264,29 @@ __device__ __forceinline__ bool sort_in_registers(int (&v)[R], int n)
     return false;
 }
 
-// Per cell, one thread: the terms of the cell's particles for each of its
-// four corner nodes (shape value and gradients times the particle's
-// coefficients), summed in ascending particle index.  Each particle is
-// loaded and its shape functions are evaluated once for the four nodes it
-// touches.  Bins are filled in lane order and are mostly sorted already;
-// bins of up to 32 entries are checked and sorted in registers (the first
-// 16 entries are loaded with the count) and read back from LDS, longer ones
-// are sorted in place.
+// Per cell, four lanes, one per corner node q: the terms of the cell's
+// particles for that node (shape value and gradients times the particle's
+// coefficients), summed in ascending particle index.  The four lanes of a
+// cell load the same particles (one request) and evaluate the shape
+// functions as shape_functions does, each keeping its corner.  Bins are
+// filled in lane order and are mostly sorted already; bins of up to 32
+// entries are checked and sorted in registers (the first 16 entries are
+// loaded with the count), longer ones are sorted by insertion; each lane
+// keeps its sorted copy in its own LDS row.
 __global__ __launch_bounds__(256) void k_adj_cells(
-    const Particles *__restrict__ Pd, const int *__restrict__ cnt, int *__restrict__ bkt,
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt, const int *__restrict__ bkt,
     AdjState A, GridParams g, int np)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
-    __shared__ int s_bin[256 * 33];   // per thread, stride 33 (no bank conflicts)
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.ncells) {
-        return;
-    }
-    const int cc = cell % g.Ne, rr = cell / g.Ne;
-    int *l = bkt + (size_t)cell * g.cap;
-    int *sb = s_bin + threadIdx.x * 33;
-    int n;
-    {
+    __shared__ int s_bin[256 * 65];   // per thread, stride 65 (no bank conflicts)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cell = t >> 2, q = t & 3;
+    const bool in = cell < g.ncells;
+    const int cc = in ? cell % g.Ne : 0, rr = in ? cell / g.Ne : 0;
+    const int *l = bkt + (size_t)(in ? cell : 0) * g.cap;
+    int *sb = s_bin + threadIdx.x * 65;
+    int n = 0;
+    if (in) {
         const int4 *l4 = reinterpret_cast<const int4 *>(l);
         int v[32];
 #pragma unroll
@@ -2321,26 +2321,24 @@ __global__ __launch_bounds__(256) void k_adj_cells(
                 sb[k] = v[k];
             }
         } else {
-            for (int k = 1; k < n; ++k) {
+            for (int k = 0; k < n; ++k) {
                 const int key = l[k];
                 int r = k - 1;
-                while (r >= 0 && l[r] > key) {
-                    l[r + 1] = l[r];
+                while (r >= 0 && sb[r] > key) {
+                    sb[r + 1] = sb[r];
                     --r;
                 }
-                l[r + 1] = key;
+                sb[r + 1] = key;
             }
         }
     }
-    const int *bin = n <= 32 ? sb : l;
-    double F[6][4];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            F[f][q] = 0.0;
-        }
+    if (!in) {
+        return;
     }
+    const int *bin = sb;
+    const double xn = (double)cc * g.h, yn = (double)rr * g.h;
+    const bool qx = q == 1 || q == 2, qy = q >= 2;
+    double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     // particles in chunks of 4: the chunk's loads are all in flight
     // together; the sums still run in index order
     for (int k0 = 0; k0 < n; k0 += 4) {
@@ -2362,28 +2360,27 @@ __global__ __launch_bounds__(256) void k_adj_cells(
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (k0 + u < n) {
-                Shape sh;
-                shape_functions(px[u], py[u], cc, rr, g, sh);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double S = sh.s[q], b1 = sh.b1[q], b2 = sh.b2[q];
-                    F[0][q] += S * cf[u][0];
-                    F[1][q] += S * cf[u][1];
-                    F[2][q] += S * cf[u][2];
-                    F[3][q] += S * cf[u][3];
-                    F[4][q] += b1 * cf[u][4] + b2 * cf[u][5];
-                    F[5][q] += b2 * cf[u][6] + b1 * cf[u][7];
-                }
+                // shape_functions' values for corner q
+                const double xl = snap_local(div_h(px[u] - xn, g));
+                const double yl = snap_local(div_h(py[u] - yn, g));
+                const double X = qx ? xl : 1.0 - xl;
+                const double Y = qy ? yl : 1.0 - yl;
+                const double S = X * Y;
+                const double b1 = (q == 0 || q == 3 ? -Y : Y) * g.inv_h;
+                const double b2 = (q <= 1 ? -X : X) * g.inv_h;
+                F[0] += S * cf[u][0];
+                F[1] += S * cf[u][1];
+                F[2] += S * cf[u][2];
+                F[3] += S * cf[u][3];
+                F[4] += b1 * cf[u][4] + b2 * cf[u][5];
+                F[5] += b2 * cf[u][6] + b1 * cf[u][7];
             }
         }
     }
     const size_t nc = (size_t)g.ncells;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            A.cellsum[(f * 4 + q) * nc + cell] = F[f][q];
-        }
+        A.cellsum[(f * 4 + q) * nc + cell] = F[f];
     }
 }
 
@@ -4346,7 +4343,7 @@ static int adj_reverse_step(mpm_hip_adjoint *a, int k)
     int *bkt = a->seg_bkt + (size_t)k * ctx->g.ncells * ctx->g.cap;
     k_adj_g2p<<<dim3(a->g2p_blocks), dim3(kG2PThreads), 0, ctx->stream>>>(
         Pk, nodes, a->A, ctx->g, ctx->mt, np);
-    k_adj_cells<<<dim3((ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
+    k_adj_cells<<<dim3((4 * ctx->g.ncells + 255) / 256), dim3(256), 0, ctx->stream>>>(
         Pk, cnt, bkt, a->A, ctx->g, np);
     k_adj_nodes<<<dim3((nn + 255) / 256), dim3(256), 0, ctx->stream>>>(
         nodes, ctx->mask, a->A, ctx->g, stk);
@@ -4411,6 +4408,7 @@ int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out)
 {
     if (!cfg || !out || checkpoint_every <= 0 || cfg->material_kind != MPM_MAT_G_LOCAL_MU2
+        || cfg->N > 23170   // four lanes per cell stay within int range
         || cfg->scheme != MPM_SCHEME_USL
         || (cfg->flags & MPM_HIP_HOST_MATERIAL)) {
         return MPM_HIP_ERR_INVALID;

@@ -120,10 +120,10 @@ def test_material_reverse_branches():
 # ---------------------------------------------------------------------------
 # GPU
 
-def gpu_config(w, material=None):
+def gpu_config(w, material=None, cap=0):
     return pympm.make_config(w["N"], w["h"], w["dt"], material=material or w["material"],
                              bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
-                             g_mag=w["g_mag"], ramp_time=w["ramp_time"])
+                             g_mag=w["g_mag"], ramp_time=w["ramp_time"], bin_capacity=cap)
 
 
 def gpu_final(w, steps, material=None, particles=None):
@@ -156,6 +156,33 @@ def test_gpu_adjoint_matches_oracle_adjoint():
         scale = np.abs(lam0[k]).max()
         err = np.abs(gi[k] - lam0[k]).max()
         assert err <= PARITY_TOL * scale, f"d/d initial {k}: max err {err} scale {scale}"
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_dense_cells_match_oracle_adjoint():
+    """36 particles per cell in bins of capacity 64: the reverse gather's
+    long-bin path (sorted by insertion) against the CPU restatement."""
+    n = 16
+    p = workloads.block(4, 6, n, seed=9, cell_x0=5, ppc_side=6)
+    p = workloads.lithostatic(p, lambda x: 6.0 / n + 0 * x, k0=0.4)
+    rng = np.random.default_rng(9)
+    p["vx"] = 0.05 * rng.standard_normal(len(p["x"]))
+    w = dict(small_case(), particles=p)
+    steps = 12
+    say(f"adjoint parity, dense cells: {len(p['x'])} particles, {steps} steps")
+    recs, final = AO.record(w, steps)
+    wts = weights(len(final["x"]), seed=4)
+    lam0, grad = AO.adjoint(recs, wts, w["material"], w["N"], w["h"], w["dt"])
+    a = pympm.Adjoint(gpu_config(w, cap=64), p, checkpoint_every=5)
+    a.forward(steps)
+    gp, gi = a.backward(wts)
+    a.close()
+    for i, k in enumerate(AO.PARAMS):
+        rel = abs(gp[k] - grad[i]) / max(abs(grad[i]), 1e-300)
+        assert rel <= PARITY_TOL, (k, gp[k], grad[i])
+    for k in ("x", "y", "vx", "vy", "v", "txx", "tyy", "gammap"):
+        scale = np.abs(lam0[k]).max()
+        assert np.abs(gi[k] - lam0[k]).max() <= PARITY_TOL * scale, k
 
 
 @pytest.mark.gpu
