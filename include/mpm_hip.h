@@ -188,7 +188,9 @@ int mpm_hip_create(const mpm_hip_config *cfg, size_t num_particles,
  * position, velocity, in-plane stress (NULL = zero).  Performs mpm_init's
  * element assignment and deactivation, g_local_mu2's material_init
  * (T from s, state zeroed) and initial_loads (zero body force), and bins the
- * particles for the first step. */
+ * particles for the first step.  The positions also set the order in which
+ * the P2G kernel starts its tiles (the tiles around the particles first); it
+ * does not affect results and is dropped when MPM_F_X or MPM_F_Y is set. */
 int mpm_hip_set_particles(mpm_hip_ctx *ctx, const double *m, const double *v,
                           const double *x, const double *y,
                           const double *vx, const double *vy,
@@ -299,7 +301,9 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b);
 typedef struct mpm_hip_adjoint mpm_hip_adjoint;
 
 /* g_local_mu2 only (no MPM_HIP_HOST_MATERIAL, no g_nonlocal), USL, and at
- * most 23170 nodes per side. */
+ * most 23170 nodes per side.  Device memory: per checkpoint-interval step,
+ * one particle state, the node records and the bins (kept by the segment
+ * recompute for the reverse steps), plus one particle state per checkpoint. */
 int mpm_hip_adjoint_create(const mpm_hip_config *cfg, size_t num_particles,
                            int checkpoint_every, mpm_hip_adjoint **out);
 /* The initial state, as mpm_hip_set_particles. */
