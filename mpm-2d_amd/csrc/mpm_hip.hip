@@ -62,6 +62,7 @@ constexpr int kTileNodes = kTileX * kTileY;
 // sums are split between them.
 constexpr int kG2PThreads = 256;
 constexpr int kBatchTileChunk = 16;   // ensemble P2G tiles per XCD chunk
+constexpr int kLightGroup = 16;       // ensemble P2G: empty-at-upload tiles per workgroup
 // Particle count limit: byte offsets into a particle field fit the 31-bit
 // range of a raw buffer load.
 constexpr size_t kMaxParticles = (size_t)1 << 28;
@@ -154,6 +155,7 @@ struct TileOrder {
 struct BatchTables {
     const MemberInfo *mem;
     const int2 *tile_order;    // P2G workgroup -> (tile, member); tile -1: a spare one
+    const int2 *light_tiles;   // tiles for k_p2g_light: (tile, member)
     const int *block_member;   // particle workgroup -> member
     DevState *st;              // per member
 };
@@ -525,15 +527,17 @@ __device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const Ti
                : -1;
 }
 
+// P2G, boundary conditions and grid update of one tile (global index
+// `tile`; member `mb` of an ensemble), by the whole workgroup.
 // MUSL2: the second mapping of the MUSL step (process_usl.c:409-440): only
 // the mass and momentum chains, then v = p / m without boundary conditions
 // or the force update.
 template <int CAP, bool BATCH, bool MUSL2, int ROLES>
-__global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
+__device__ __forceinline__ void p2g_tile_body(
     const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
     int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
     const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
-    int tiles_x, int ntiles, int np, TileOrder ord, BatchTables bt)
+    int tiles_x, int np, const BatchTables &bt, int tile, int mb)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
     constexpr int kP2GThreads = ROLES * kTileNodes;
@@ -564,18 +568,6 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     __shared__ double2 s_pxs[kStageMax];    // vx * m, -sxx * v
     __shared__ double2 s_pys[kStageMax];    // vy * m, -syy * v
 
-    // An ensemble's tiles come from the dispatch table built on the host.
-    int tile, mb = 0;
-    if constexpr (BATCH) {
-        const int2 tm = bt.tile_order[blockIdx.x];
-        tile = tm.x;
-        mb = tm.y;
-    } else {
-        tile = p2g_tile(blockIdx.x, tiles_x, ntiles, ord);
-    }
-    if (tile < 0 || tile >= ntiles) {
-        return;
-    }
     // 1: the tile's node records are all +0 (written so by the last launch,
     // or never written since the array was zeroed)
     const int zero_before = tile_zero[tile];
@@ -1024,6 +1016,89 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
     r.vy = nvy;
     r.pad = 0.0;
     nodes[(size_t)j * g.N + i] = r;
+}
+
+// k_p2g_grid: p2g_tile_body for one tile per workgroup: a single job's from
+// p2g_tile, an ensemble's from the host-built dispatch table (BatchTables).
+template <int CAP, bool BATCH, bool MUSL2, int ROLES>
+__global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
+    const uint8_t *__restrict__ mask, GridParams g, const DevState *__restrict__ st,
+    int tiles_x, int ntiles, int np, TileOrder ord, BatchTables bt)
+{
+    int tile, mb = 0;
+    if constexpr (BATCH) {
+        const int2 tm = bt.tile_order[blockIdx.x];
+        tile = tm.x;
+        mb = tm.y;
+    } else {
+        tile = p2g_tile(blockIdx.x, tiles_x, ntiles, ord);
+    }
+    if (tile < 0 || tile >= ntiles) {
+        return;
+    }
+    p2g_tile_body<CAP, BATCH, MUSL2, ROLES>(Pd, cnt_cur, bkt_cur, cnt_next, nodes, tile_zero, mask,
+                                            g, st, tiles_x, np, bt, tile, mb);
+}
+
+// The ensemble's light tiles (regions without particles at upload),
+// kLightGroup per workgroup from BatchTables::light_tiles: the group's bin
+// counts are checked in one pass (and its own cells' next-step counts
+// cleared); p2g_tile_body runs only for tiles that now hold particles or
+// whose node records are not known to be zero.  (A kernel of its own: the
+// loop around the body needs more registers than k_p2g_grid can spare.)
+template <int CAP>
+__global__ __launch_bounds__(2 * kTileNodes) void k_p2g_light(
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
+    int np, int n_light, BatchTables bt)
+{
+    constexpr int kThreads = 2 * kTileNodes;
+    __shared__ int4 s_geo[kLightGroup];   // region origin, grid side, cell base
+    __shared__ int s_need[kLightGroup];   // the body must run for the tile
+    __shared__ int2 s_tm[kLightGroup];
+    const int e0 = blockIdx.x * kLightGroup;
+    const int n = min(kLightGroup, n_light - e0);
+    if (n <= 0) {
+        return;
+    }
+    const int tid = threadIdx.x;
+    if (tid < n) {
+        const int2 tm = bt.light_tiles[e0 + tid];
+        const MemberInfo &mi = bt.mem[tm.y];
+        const int lt = tm.x - mi.tile_base;
+        s_geo[tid] = make_int4((lt % mi.tiles_x) * kTileX, (lt / mi.tiles_x) * kTileY, mi.g.Ne,
+                               mi.cell_base);
+        s_need[tid] = !tile_zero[tm.x];
+        s_tm[tid] = tm;
+    }
+    __syncthreads();
+    for (int it = tid; it < n * kRegCells; it += kThreads) {
+        const int u = it / kRegCells, cl = it % kRegCells;
+        const int4 geo = s_geo[u];
+        const int ax = cl % kRegX, ay = cl / kRegX;
+        const int cc = geo.x - 1 + ax, rr = geo.y - 1 + ay;
+        if (cc >= 0 && rr >= 0 && cc < geo.z && rr < geo.z) {
+            const int c = geo.w + rr * geo.z + cc;
+            if (cnt_cur[c] > 0) {
+                s_need[u] = 1;
+            }
+            if (ax > 0 && ay > 0) {   // the tile's own cell
+                cnt_next[c] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    const GridParams g0{};
+    for (int u = 0; u < n; ++u) {
+        if (s_need[u]) {
+            p2g_tile_body<CAP, true, false, 2>(Pd, cnt_cur, bkt_cur, cnt_next, nodes, tile_zero,
+                                               nullptr, g0, nullptr, 0, np, bt, s_tm[u].x,
+                                               s_tm[u].y);
+            __syncthreads();   // LDS is reused by the next tile
+        }
+    }
 }
 
 template <bool BATCH>
@@ -2926,10 +3001,13 @@ struct mpm_hip_batch {
     unsigned char *have;       // particles set, per member
     MemberInfo *dmem;
     int2 *tile_order;          // P2G workgroup -> (tile, member), -1 tile: spare
+    int2 *light_tiles;         // k_p2g_light's tiles: (tile, member)
     int *block_member;
     int *htile_member;         // host: tile -> member
     int *tile_load;            // host: particles in each tile's cell region
-    int2 *horder;              // host copy of tile_order
+    int2 *horder, *hlight;     // host copies of tile_order and light_tiles
+    int max_p2g_blocks;        // tile_order capacity
+    int n_light;               // light tiles
     bool order_dirty;          // tile_load changed since tile_order was built
     DevState *st;
     double *pfield[MPM_F_EXX];
@@ -2950,29 +3028,42 @@ static BatchTables batch_tables(const mpm_hip_batch *b)
     BatchTables t;
     t.mem = b->dmem;
     t.tile_order = b->tile_order;
+    t.light_tiles = b->light_tiles;
     t.block_member = b->block_member;
     t.st = b->st;
     return t;
 }
 
-static int batch_launch_p2g(mpm_hip_batch *b, int cur)
+template <int CAP>
+static void batch_p2g_kernels(mpm_hip_batch *b, int cur)
 {
     const int nxt = 1 - cur;
-    const dim3 grid(b->p2g_blocks), block(2 * kTileNodes);
     const BatchTables t = batch_tables(b);
     const GridParams g0{};
+    const int np = (int)b->np_total;
+    if (b->p2g_blocks) {
+        k_p2g_grid<CAP, true, false, 2><<<dim3(b->p2g_blocks), dim3(2 * kTileNodes), 0, b->stream>>>(
+            b->dP, b->cnt[cur], b->bkt[cur], b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0,
+            nullptr, 0, b->ntiles, np, TileOrder{}, t);
+    }
+    if (b->n_light) {
+        const int blocks = (b->n_light + kLightGroup - 1) / kLightGroup;
+        k_p2g_light<CAP><<<dim3(blocks), dim3(2 * kTileNodes), 0, b->stream>>>(
+            b->dP, b->cnt[cur], b->bkt[cur], b->cnt[nxt], b->nodes, b->tile_zero, np, b->n_light, t);
+    }
+}
+
+static int batch_launch_p2g(mpm_hip_batch *b, int cur)
+{
     switch (b->cap) {
     case 16:
-        k_p2g_grid<16, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
+        batch_p2g_kernels<16>(b, cur);
         break;
     case 32:
-        k_p2g_grid<32, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
+        batch_p2g_kernels<32>(b, cur);
         break;
     default:
-        k_p2g_grid<64, true, false, 2><<<grid, block, 0, b->stream>>>(b->dP, b->cnt[cur], b->bkt[cur],
-            b->cnt[nxt], b->nodes, b->tile_zero, nullptr, g0, nullptr, 0, b->ntiles, (int)b->np_total, TileOrder{}, t);
+        batch_p2g_kernels<64>(b, cur);
         break;
     }
     HIP_TRY(hipGetLastError());
@@ -3755,7 +3846,9 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     b->nn_total = nbase;
     b->ncells_total = cbase;
     b->ntiles = (int)tbase;
-    b->p2g_blocks = p2g_grid_blocks(b->ntiles, kBatchTileChunk);
+    // occupied tiles one per workgroup in whole chunk rounds (batch_build_order)
+    b->max_p2g_blocks = p2g_grid_blocks(b->ntiles, kBatchTileChunk);
+    b->p2g_blocks = 0;
     b->g2p_blocks = (int)(pbase / kG2PThreads);
 
     if (hipSetDevice(device) != hipSuccess
@@ -3801,7 +3894,8 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     }
     BALLOC(b->dmem, (size_t)num_members * sizeof(MemberInfo));
     BALLOC(b->st, (size_t)num_members * sizeof(DevState));
-    BALLOC(b->tile_order, (size_t)b->p2g_blocks * sizeof(int2));
+    BALLOC(b->tile_order, (size_t)b->max_p2g_blocks * sizeof(int2));
+    BALLOC(b->light_tiles, (size_t)b->ntiles * sizeof(int2));
     BALLOC(b->block_member, (size_t)b->g2p_blocks * sizeof(int));
     BALLOC(b->dP, sizeof(Particles));
 #undef BALLOC
@@ -3812,11 +3906,12 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
 
     b->htile_member = new (std::nothrow) int[b->ntiles ? b->ntiles : 1];
     b->tile_load = new (std::nothrow) int[b->ntiles ? b->ntiles : 1]();
-    b->horder = new (std::nothrow) int2[b->p2g_blocks ? b->p2g_blocks : 1];
+    b->horder = new (std::nothrow) int2[b->max_p2g_blocks];
+    b->hlight = new (std::nothrow) int2[b->ntiles ? b->ntiles : 1];
     int *tm = b->htile_member;
     int *bm = (int *)malloc(sizeof(int) * (size_t)b->g2p_blocks);
     DevState *s0 = (DevState *)calloc((size_t)num_members, sizeof(DevState));
-    if (!tm || !b->tile_load || !b->horder || !bm || !s0) {
+    if (!tm || !b->tile_load || !b->horder || !b->hlight || !bm || !s0) {
         free(bm);
         free(s0);
         mpm_hip_batch_destroy(b);
@@ -3887,21 +3982,21 @@ static void batch_tile_loads(mpm_hip_batch *b, int k, const double *x, const dou
     b->order_dirty = true;
 }
 
-// The ensemble's P2G dispatch table (BatchTables::tile_order): tiles carrying
-// at least half the load of their member's heaviest tile first, then the
-// other tiles that hold particles, then the empty ones, each class in tile
-// order, so the heavy tiles start first and are dealt evenly to the CUs;
-// chunks of kBatchTileChunk consecutive entries go to the 8 XCDs in turn.
-// The order is a hint: every tile is in the table once, and any order gives
-// the same node sums.
+// The ensemble's P2G dispatch tables (BatchTables): tiles carrying at least
+// half the load of their member's heaviest tile first, then the other tiles
+// that hold particles, one per k_p2g_grid workgroup (chunks of
+// kBatchTileChunk to the 8 XCDs in turn), so the heavy tiles start first and
+// are dealt evenly to the CUs; the tiles whose regions held no particle go
+// to k_p2g_light, kLightGroup per workgroup.  The order is a hint: every
+// tile is in one table once, and any order gives the same node sums.
 static int batch_build_order(mpm_hip_batch *b)
 {
     const int nt = b->ntiles;
-    int *seq = new (std::nothrow) int[nt ? nt : 1];
     int *heavy = new (std::nothrow) int[b->nmem];
-    if (!seq || !heavy) {
-        delete[] seq;
+    int *seq = new (std::nothrow) int[nt ? nt : 1];
+    if (!heavy || !seq) {
         delete[] heavy;
+        delete[] seq;
         return MPM_HIP_ERR_ALLOC;
     }
     for (int k = 0; k < b->nmem; ++k) heavy[k] = 0;
@@ -3909,27 +4004,37 @@ static int batch_build_order(mpm_hip_batch *b)
         int &h = heavy[b->htile_member[t]];
         h = std::max(h, b->tile_load[t]);
     }
-    auto cls = [&](int t) {
-        const int l = b->tile_load[t];
-        return l == 0 ? 2 : (2 * l >= heavy[b->htile_member[t]] ? 0 : 1);
-    };
-    int n = 0;
-    for (int c = 0; c < 3; ++c) {
+    int n_occ = 0, n_light = 0;
+    for (int c = 0; c < 2; ++c) {
         for (int t = 0; t < nt; ++t) {
-            if (cls(t) == c) seq[n++] = t;
+            const int l = b->tile_load[t];
+            if (l > 0 && (2 * l >= heavy[b->htile_member[t]]) == (c == 0)) seq[n_occ++] = t;
         }
     }
+    for (int t = 0; t < nt; ++t) {
+        if (b->tile_load[t] == 0) b->hlight[n_light++] = make_int2(t, b->htile_member[t]);
+    }
     constexpr int C = kBatchTileChunk;
-    for (int blk = 0; blk < b->p2g_blocks; ++blk) {
+    const int blocks = n_occ ? p2g_grid_blocks(n_occ, C) : 0;
+    for (int blk = 0; blk < blocks; ++blk) {
         const int xcd = blk & 7, slot = blk >> 3;
         const int v = ((slot / C) * 8 + xcd) * C + slot % C;
-        b->horder[blk] = v < nt ? make_int2(seq[v], b->htile_member[seq[v]]) : make_int2(-1, -1);
+        b->horder[blk] = v < n_occ ? make_int2(seq[v], b->htile_member[seq[v]]) : make_int2(-1, -1);
     }
-    delete[] seq;
     delete[] heavy;
-    HIP_TRY(hipMemcpyAsync(b->tile_order, b->horder, sizeof(int2) * (size_t)b->p2g_blocks,
-                           hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));   // horder is rewritten by the next build
+    delete[] seq;
+    batch_drop_graph(b);   // the captured launches have the old grids
+    b->p2g_blocks = blocks;
+    b->n_light = n_light;
+    if (blocks) {
+        HIP_TRY(hipMemcpyAsync(b->tile_order, b->horder, sizeof(int2) * (size_t)blocks,
+                               hipMemcpyHostToDevice, b->stream));
+    }
+    if (n_light) {
+        HIP_TRY(hipMemcpyAsync(b->light_tiles, b->hlight, sizeof(int2) * (size_t)n_light,
+                               hipMemcpyHostToDevice, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));   // the host copies are rewritten by the next build
     b->order_dirty = false;
     return MPM_HIP_OK;
 }
@@ -4227,7 +4332,8 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
         if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
     void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->tile_zero, b->cnt[0], b->cnt[1], b->bkt[0],
-                    b->bkt[1], b->dmem, b->st, b->tile_order, b->block_member, b->dP};
+                    b->bkt[1], b->dmem, b->st, b->tile_order, b->light_tiles, b->block_member,
+                    b->dP};
     for (void *a : arrs) {
         if (a) (void)hipFree(a);
     }
@@ -4236,6 +4342,7 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
     delete[] b->htile_member;
     delete[] b->tile_load;
     delete[] b->horder;
+    delete[] b->hlight;
     delete[] b->cfgs;
     delete[] b->have;
     delete b;
