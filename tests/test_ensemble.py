@@ -106,6 +106,36 @@ def test_batch_matches_single_contexts():
 
 
 @pytest.mark.gpu
+def test_batch_particles_crossing_into_empty_tiles():
+    """Members whose blocks fly across tile rows and columns: tiles whose
+    regions were empty at upload (checked sixteen at a time by the light-tile
+    kernel) fill up and empty again, and tiles occupied at upload empty out;
+    every member matches the oracle bitwise."""
+    members = []
+    for k, (n, cx0, cy0, vx, vy) in enumerate([(48, 4, 20, 0.4, -0.1), (64, 30, 30, -0.35, 0.2),
+                                               (40, 2, 2, 0.3, 0.3)]):
+        w = workloads.small_box(cells=n, bx=6, by=5, steps=0)
+        p = workloads.block(6, 5, n, seed=20 + k, cell_x0=cx0, cell_y0=cy0)
+        p["vx"][:] = vx * w["h"] / w["dt"]      # cells per step
+        p["vy"][:] = vy * w["h"] / w["dt"]
+        members.append(dict(w, particles=p))
+    steps = 50
+    say(f"batch of {len(members)} crossing tiles, {steps} steps")
+    b = pympm.Batch([member_config(w) for w in members], [w["particles"] for w in members])
+    b.step(steps)
+    b.sync()
+    for k, w in enumerate(members):
+        o = pyoracle.from_workload(w, num_threads=2)
+        o.run(steps)
+        ref = o.particles()
+        assert_member_equal(b, k, ref, ref["active"], "crossing tiles")
+        moved = np.abs(ref["x"] - w["particles"]["x"]).min() * (w["N"] - 1)
+        assert moved > 12.0, f"member {k} moved only {moved} cells"
+        o.close()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_autotune_keeps_the_trajectory():
     ens = workloads.ensemble(members=10, seed=8, p_max=8192)
     say(f"autotune on a batch of {len(ens)}")
