@@ -62,7 +62,8 @@ constexpr int kTileNodes = kTileX * kTileY;
 // sums are split between them.
 constexpr int kG2PThreads = 256;
 constexpr int kBatchTileChunk = 16;   // ensemble P2G tiles per XCD chunk
-constexpr int kLightGroup = 16;       // ensemble P2G: empty-at-upload tiles per workgroup
+constexpr int kLightGroup = 32;       // ensemble P2G: empty-at-upload tiles per workgroup
+constexpr int kFixBlocks = 256;       // ensemble P2G: workgroups for listed light tiles
 // Particle count limit: byte offsets into a particle field fit the 31-bit
 // range of a raw buffer load.
 constexpr size_t kMaxParticles = (size_t)1 << 28;
@@ -476,12 +477,12 @@ __global__ __launch_bounds__(kG2PThreads) void k_bin_particles(
     bin_insert(e >= 0, e, p, cnt, bkt, g.cap, &st->status);
 }
 
-// The v-th tile, in row-major order, of the x0.., y0.., w x h tiles that lie
-// outside the inner rectangle ix0.., iy0.., iw x ih within them.
-__device__ __forceinline__ int rect_rim_tile(int v, int x0, int y0, int w, int h, int ix0,
-                                             int iy0, int iw, int ih, int tiles_x)
+// The v-th tile, in row-major order, of the tiles of the rectangle x0.., y0..
+// (w tiles wide) that lie outside the inner rectangle ix0.., iy0.., iw x ih
+// within it.
+__device__ __forceinline__ int rect_rim_tile(int v, int x0, int y0, int w, int ix0, int iy0,
+                                             int iw, int ih, int tiles_x)
 {
-    (void)h;
     const int n0 = (iy0 - y0) * w;   // rows below the inner rectangle
     if (v < n0) {
         return (y0 + v / w) * tiles_x + x0 + v % w;
@@ -517,13 +518,13 @@ __device__ __forceinline__ int p2g_tile(int b, int tiles_x, int ntiles, const Ti
     if (b < o.blocks_b) {
         const int v = chunked(b, 1);
         return v < o.bw * o.bh - o.cw * o.ch
-                   ? rect_rim_tile(v, o.bx0, o.by0, o.bw, o.bh, o.cx0, o.cy0, o.cw, o.ch, tiles_x)
+                   ? rect_rim_tile(v, o.bx0, o.by0, o.bw, o.cx0, o.cy0, o.cw, o.ch, tiles_x)
                    : -1;
     }
     b -= o.blocks_b;
     const int v = chunked(b, tiles_x);
     return v < ntiles - o.bw * o.bh
-               ? rect_rim_tile(v, 0, 0, tiles_x, ntiles / tiles_x, o.bx0, o.by0, o.bw, o.bh, tiles_x)
+               ? rect_rim_tile(v, 0, 0, tiles_x, o.bx0, o.by0, o.bw, o.bh, tiles_x)
                : -1;
 }
 
@@ -1044,19 +1045,16 @@ __global__ __launch_bounds__(ROLES * kTileNodes) void k_p2g_grid(
 
 // The ensemble's light tiles (regions without particles at upload),
 // kLightGroup per workgroup from BatchTables::light_tiles: the group's bin
-// counts are checked in one pass (and its own cells' next-step counts
-// cleared); p2g_tile_body runs only for tiles that now hold particles or
-// whose node records are not known to be zero.  (A kernel of its own: the
-// loop around the body needs more registers than k_p2g_grid can spare.)
-template <int CAP>
-__global__ __launch_bounds__(2 * kTileNodes) void k_p2g_light(
-    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
-    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
-    int np, int n_light, BatchTables bt)
+// counts are checked in one pass and its own cells' next-step counts
+// cleared; a tile that now holds particles, or whose node records are not
+// known to be zero, is appended to fix_list for k_p2g_fix.
+__global__ __launch_bounds__(256) void k_p2g_light(
+    const int *__restrict__ cnt_cur, int *__restrict__ cnt_next,
+    const uint8_t *__restrict__ tile_zero, int n_light, BatchTables bt, int *__restrict__ fix_count,
+    int2 *__restrict__ fix_list)
 {
-    constexpr int kThreads = 2 * kTileNodes;
     __shared__ int4 s_geo[kLightGroup];   // region origin, grid side, cell base
-    __shared__ int s_need[kLightGroup];   // the body must run for the tile
+    __shared__ int s_need[kLightGroup];   // the tile needs p2g_tile_body
     __shared__ int2 s_tm[kLightGroup];
     const int e0 = blockIdx.x * kLightGroup;
     const int n = min(kLightGroup, n_light - e0);
@@ -1074,7 +1072,7 @@ __global__ __launch_bounds__(2 * kTileNodes) void k_p2g_light(
         s_tm[tid] = tm;
     }
     __syncthreads();
-    for (int it = tid; it < n * kRegCells; it += kThreads) {
+    for (int it = tid; it < n * kRegCells; it += 256) {
         const int u = it / kRegCells, cl = it % kRegCells;
         const int4 geo = s_geo[u];
         const int ax = cl % kRegX, ay = cl / kRegX;
@@ -1090,14 +1088,32 @@ __global__ __launch_bounds__(2 * kTileNodes) void k_p2g_light(
         }
     }
     __syncthreads();
+    if (tid < n && s_need[tid]) {
+        fix_list[atomicAdd(fix_count, 1)] = s_tm[tid];
+    }
+}
+
+// p2g_tile_body for the tiles k_p2g_light listed (fix_count of them, in
+// list order over a fixed grid; usually none), and the other step parity's
+// list emptied for the next step.  (A kernel of its own: the loop around
+// the body needs more registers than k_p2g_grid can spare.)
+template <int CAP>
+__global__ __launch_bounds__(2 * kTileNodes) void k_p2g_fix(
+    const Particles *__restrict__ Pd, const int *__restrict__ cnt_cur, const int *__restrict__ bkt_cur,
+    int *__restrict__ cnt_next, NodeRec *__restrict__ nodes, uint8_t *__restrict__ tile_zero,
+    int np, const int *__restrict__ fix_count, const int2 *__restrict__ fix_list,
+    int *__restrict__ next_count, BatchTables bt)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *next_count = 0;
+    }
+    const int n = *fix_count;
     const GridParams g0{};
-    for (int u = 0; u < n; ++u) {
-        if (s_need[u]) {
-            p2g_tile_body<CAP, true, false, 2>(Pd, cnt_cur, bkt_cur, cnt_next, nodes, tile_zero,
-                                               nullptr, g0, nullptr, 0, np, bt, s_tm[u].x,
-                                               s_tm[u].y);
-            __syncthreads();   // LDS is reused by the next tile
-        }
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        const int2 tm = fix_list[e];
+        p2g_tile_body<CAP, true, false, 2>(Pd, cnt_cur, bkt_cur, cnt_next, nodes, tile_zero, nullptr,
+                                           g0, nullptr, 0, np, bt, tm.x, tm.y);
+        __syncthreads();   // LDS is reused by the next tile
     }
 }
 
@@ -2076,7 +2092,7 @@ struct AdjState {
     double *coef;               // [kAdjCoef][np]
     double *nadj;               // [5][nn]: m~, px~, py~, fx~, fy~
     double *cellsum;            // [6][4][cells]: per cell and corner, the sums of its
-                                // particles' terms (Fx, Fy, Ux, Uy, Vx, Vy) in bin order
+                                // particles' terms (Fx, Fy, Ux, Uy, Vx, Vy) in index order
     double *partials;           // [kAdjParams][blocks]
 };
 
@@ -3008,6 +3024,8 @@ struct mpm_hip_batch {
     int2 *horder, *hlight;     // host copies of tile_order and light_tiles
     int max_p2g_blocks;        // tile_order capacity
     int n_light;               // light tiles
+    int *fix_count;            // [2] per step parity: light tiles listed for k_p2g_fix
+    int2 *fix_list;            // [2][light tiles]
     bool order_dirty;          // tile_load changed since tile_order was built
     DevState *st;
     double *pfield[MPM_F_EXX];
@@ -3048,8 +3066,14 @@ static void batch_p2g_kernels(mpm_hip_batch *b, int cur)
     }
     if (b->n_light) {
         const int blocks = (b->n_light + kLightGroup - 1) / kLightGroup;
-        k_p2g_light<CAP><<<dim3(blocks), dim3(2 * kTileNodes), 0, b->stream>>>(
-            b->dP, b->cnt[cur], b->bkt[cur], b->cnt[nxt], b->nodes, b->tile_zero, np, b->n_light, t);
+        int *count = b->fix_count + cur;
+        int2 *list = b->fix_list + (size_t)cur * b->ntiles;
+        k_p2g_light<<<dim3(blocks), dim3(256), 0, b->stream>>>(
+            b->cnt[cur], b->cnt[nxt], b->tile_zero, b->n_light, t, count, list);
+        const int fix_blocks = std::min(b->n_light, kFixBlocks);
+        k_p2g_fix<CAP><<<dim3(fix_blocks), dim3(2 * kTileNodes), 0, b->stream>>>(
+            b->dP, b->cnt[cur], b->bkt[cur], b->cnt[nxt], b->nodes, b->tile_zero, np, count, list,
+            b->fix_count + nxt, t);
     }
 }
 
@@ -3129,6 +3153,7 @@ static int batch_rebin(mpm_hip_batch *b, int init)
 {
     HIP_TRY(hipMemsetAsync(b->cnt[0], 0, sizeof(int) * b->ncells_total, b->stream));
     HIP_TRY(hipMemsetAsync(b->cnt[1], 0, sizeof(int) * b->ncells_total, b->stream));
+    HIP_TRY(hipMemsetAsync(b->fix_count, 0, 2 * sizeof(int), b->stream));
     b->cur = 0;
     const GridParams g0{};
     k_bin_particles<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
@@ -3896,6 +3921,8 @@ int mpm_hip_batch_create(const mpm_hip_config *cfgs, const size_t *num_particles
     BALLOC(b->st, (size_t)num_members * sizeof(DevState));
     BALLOC(b->tile_order, (size_t)b->max_p2g_blocks * sizeof(int2));
     BALLOC(b->light_tiles, (size_t)b->ntiles * sizeof(int2));
+    BALLOC(b->fix_count, 2 * sizeof(int));
+    BALLOC(b->fix_list, 2 * (size_t)b->ntiles * sizeof(int2));
     BALLOC(b->block_member, (size_t)b->g2p_blocks * sizeof(int));
     BALLOC(b->dP, sizeof(Particles));
 #undef BALLOC
@@ -4332,8 +4359,8 @@ int mpm_hip_batch_destroy(mpm_hip_batch *b)
         if (b->pfield[f]) (void)hipFree(b->pfield[f]);
     }
     void *arrs[] = {b->P.elem, b->P.active, b->nodes, b->tile_zero, b->cnt[0], b->cnt[1], b->bkt[0],
-                    b->bkt[1], b->dmem, b->st, b->tile_order, b->light_tiles, b->block_member,
-                    b->dP};
+                    b->bkt[1], b->dmem, b->st, b->tile_order, b->light_tiles, b->fix_count,
+                    b->fix_list, b->block_member, b->dP};
     for (void *a : arrs) {
         if (a) (void)hipFree(a);
     }
