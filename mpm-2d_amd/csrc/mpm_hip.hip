@@ -1117,14 +1117,19 @@ __global__ __launch_bounds__(2 * kTileNodes) void k_p2g_fix(
     }
 }
 
-template <bool BATCH>
-__global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
+// FIXED >= 0: the mode is that compile-time constant (the plain USL step's
+// instantiation carries no other mode's code or registers); -1: `mode`.
+template <bool BATCH, int FIXED>
+__global__ __launch_bounds__(kG2PThreads, FIXED >= 0 ? 5 : 1) void k_g2p_stress(
     const Particles *__restrict__ Pd, const NodeRec *__restrict__ nodes, int *__restrict__ cnt_next,
     int *__restrict__ bkt_next, GridParams g, Material mt, DevState *__restrict__ st,
     int np, int mode, BatchTables bt)
 {
     const Particles &P = *Pd;   // field pointers, loaded as used
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (FIXED >= 0) {
+        mode = FIXED;
+    }
     int first = 0;
     if constexpr (BATCH) {
         const int mb = bt.block_member[blockIdx.x];
@@ -1154,26 +1159,19 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     if (p >= np) {
         return;
     }
-    // Every per-particle load is issued here, before any is used: one
-    // memory round trip for the particle, then one for its four nodes.
+    // Loads in three waves, each issued before any of it is used: the
+    // particle's kinematics and its nodes' masses, momenta and forces (the
+    // move); the nodes' velocities (the strain rate); the mass, volume,
+    // stress and state (the stress update).  Each group is dead before the
+    // next is live, which keeps the plain step at 96 VGPRs (5 waves per
+    // SIMD) without spills.
     const int act = P.active[p];
     const int e = P.elem[p];
     double x = P.x[p], y = P.y[p];
     const double vx0 = P.vx[p], vy0 = P.vy[p];
     const double ux0 = P.ux[p], uy0 = P.uy[p];
-    const double mp = P.m[p], v0 = P.v[p];
     const bool stress_in = !(mode & kModeHostMaterial) || (mode & kModeNonlocal);
     const bool state_in = !(mode & (kModeHostMaterial | kModeNonlocal | kModeIsolin));
-    double sxx0 = 0.0, sxy0 = 0.0, syy0 = 0.0, szz0 = 0.0, gp0 = 0.0;
-    if (finish && stress_in) {
-        sxx0 = P.txx[p];
-        sxy0 = P.txy[p];
-        syy0 = P.tyy[p];
-    }
-    if (finish && state_in) {
-        szz0 = P.tzz[p];
-        gp0 = P.gp[p];
-    }
     double xn0 = 0.0, yn0 = 0.0;
     if (!move) {
         xn0 = P.xn[p];
@@ -1190,15 +1188,33 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
     }
     const int cc = e % g.Ne, rr = e / g.Ne;
     const double dt = g.dt;
+    // shape_functions, the values now and the gradients after the move
+    // (the same operations, fewer registers live)
     Shape sh;
-    shape_functions(x, y, cc, rr, g, sh);
+    {
+        const double xl = snap_local(div_h(x - (double)cc * g.h, g));
+        const double yl = snap_local(div_h(y - (double)rr * g.h, g));
+        const double ox = 1.0 - xl, oy = 1.0 - yl;
+        sh.s[0] = ox * oy;
+        sh.s[1] = xl * oy;
+        sh.s[2] = xl * yl;
+        sh.s[3] = ox * yl;
+        sh.xl = xl;
+        sh.yl = yl;
+    }
 
     const size_t n0 = (size_t)rr * g.N + cc;
     const size_t nid[4] = {n0, n0 + 1, n0 + g.N + 1, n0 + g.N};
-    NodeRec nd[4];
+    struct NodeMove { double m, mx, my, fx, fy; };
+    NodeMove nd[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        nd[q] = nodes[nid[q]];
+        const NodeRec *r = nodes + nid[q];
+        nd[q].m = r->m;
+        nd[q].mx = r->mx;
+        nd[q].my = r->my;
+        nd[q].fx = r->fx;
+        nd[q].fy = r->fy;
     }
 
     // move_particles_explicit_usl_split
@@ -1251,17 +1267,76 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         uy = uy0;
     }
 
+    // The new kinematics, and the next step's create_particle_to_element_map
+    // and bin: none is needed by the strain rate and stress below.
+    P.x[p] = x;
+    P.y[p] = y;
+    P.vx[p] = vx;
+    P.vy[p] = vy;
+    P.ux[p] = ux;
+    P.uy[p] = uy;
+    if ((mode & kModeStoreAux) && move) {
+        P.ax[p] = ax;
+        P.ay[p] = ay;
+        P.xl[p] = sh.xl;
+        P.yl[p] = sh.yl;
+    }
+    const int en = which_element(x, y, g);
+    P.elem[p] = en;
+    const unsigned long long crossed = __ballot(en != e);
+    if (crossed && __lane_id() == __builtin_ctzll(__ballot(1))) {
+        atomicAdd(&st->crossings, (unsigned long long)__popcll(crossed));
+    }
+    bin_insert(en >= 0, en, p, cnt_next, bkt_next, g.cap, &st->status);
+
+    {
+        const double xl = sh.xl, yl = sh.yl;
+        const double ox = 1.0 - xl, oy = 1.0 - yl;
+        sh.b1[0] = -oy * g.inv_h;
+        sh.b1[1] = oy * g.inv_h;
+        sh.b1[2] = yl * g.inv_h;
+        sh.b1[3] = -yl * g.inv_h;
+        sh.b2[0] = -ox * g.inv_h;
+        sh.b2[1] = -xl * g.inv_h;
+        sh.b2[2] = xl * g.inv_h;
+        sh.b2[3] = ox * g.inv_h;
+    }
+    // the node velocities, loaded after the move (fewer registers live)
+    if constexpr (FIXED >= 0) {
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    struct NodeVel { double vx, vy; };
+    NodeVel nv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        nv[q].vx = nodes[nid[q]].vx;
+        nv[q].vy = nodes[nid[q]].vy;
+    }
     // calculate_strainrate_split (SMEAR4 folds left to right)
-    const double exx = 1.0 * (sh.b1[0] * nd[0].vx + sh.b1[1] * nd[1].vx
-                              + sh.b1[2] * nd[2].vx + sh.b1[3] * nd[3].vx);
-    const double eyy = 1.0 * (sh.b2[0] * nd[0].vy + sh.b2[1] * nd[1].vy
-                              + sh.b2[2] * nd[2].vy + sh.b2[3] * nd[3].vy);
-    const double dx_tdy = 1.0 * (sh.b2[0] * nd[0].vx + sh.b2[1] * nd[1].vx
-                                 + sh.b2[2] * nd[2].vx + sh.b2[3] * nd[3].vx);
-    const double dy_tdx = 1.0 * (sh.b1[0] * nd[0].vy + sh.b1[1] * nd[1].vy
-                                 + sh.b1[2] * nd[2].vy + sh.b1[3] * nd[3].vy);
+    const double exx = 1.0 * (sh.b1[0] * nv[0].vx + sh.b1[1] * nv[1].vx
+                              + sh.b1[2] * nv[2].vx + sh.b1[3] * nv[3].vx);
+    const double eyy = 1.0 * (sh.b2[0] * nv[0].vy + sh.b2[1] * nv[1].vy
+                              + sh.b2[2] * nv[2].vy + sh.b2[3] * nv[3].vy);
+    const double dx_tdy = 1.0 * (sh.b2[0] * nv[0].vx + sh.b2[1] * nv[1].vx
+                                 + sh.b2[2] * nv[2].vx + sh.b2[3] * nv[3].vx);
+    const double dy_tdx = 1.0 * (sh.b1[0] * nv[0].vy + sh.b1[1] * nv[1].vy
+                                 + sh.b1[2] * nv[2].vy + sh.b1[3] * nv[3].vy);
     const double exy = 0.5 * (dx_tdy + dy_tdx);
     const double wxy = 0.5 * (dx_tdy - dy_tdx);
+    if constexpr (FIXED >= 0) {
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const double mp = P.m[p], v0 = P.v[p];
+    double sxx0 = 0.0, sxy0 = 0.0, syy0 = 0.0, szz0 = 0.0, gp0 = 0.0;
+    if (finish && stress_in) {
+        sxx0 = P.txx[p];
+        sxy0 = P.txy[p];
+        syy0 = P.tyy[p];
+    }
+    if (finish && state_in) {
+        szz0 = P.tzz[p];
+        gp0 = P.gp[p];
+    }
 
     // update_particle_densities_split
     const double v = v0 * mpm_exp(dt * (exx + eyy));
@@ -1345,34 +1420,13 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_stress(
         P.gp[p] = gp;
         P.gdp[p] = gdp;
     }
-    P.x[p] = x;
-    P.y[p] = y;
-    P.vx[p] = vx;
-    P.vy[p] = vy;
     P.v[p] = v;
-    P.ux[p] = ux;
-    P.uy[p] = uy;
     if (mode & kModeStoreAux) {
         P.exx[p] = exx;
         P.exy[p] = exy;
         P.eyy[p] = eyy;
         P.wxy[p] = wxy;
-        if (move) {
-            P.ax[p] = ax;
-            P.ay[p] = ay;
-            P.xl[p] = sh.xl;
-            P.yl[p] = sh.yl;
-        }
     }
-
-    // The next step's create_particle_to_element_map, and its bin.
-    const int en = which_element(x, y, g);
-    P.elem[p] = en;
-    const unsigned long long crossed = __ballot(en != e);
-    if (crossed && __lane_id() == __builtin_ctzll(__ballot(1))) {
-        atomicAdd(&st->crossings, (unsigned long long)__popcll(crossed));
-    }
-    bin_insert(en >= 0, en, p, cnt_next, bkt_next, g.cap, &st->status);
 }
 
 // ---------------------------------------------------------------------------
@@ -2982,15 +3036,21 @@ static int launch_g2p(mpm_hip_ctx *ctx)
     if (ctx->cfg.scheme == MPM_SCHEME_MUSL) {
         // particle update; second mapping of the momentum; strain rate,
         // volume, stress and rebinning from the re-mapped velocities
-        k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+        k_g2p_stress<false, -1><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
             ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
             (int)ctx->np, mode | kModeMuslMove, BatchTables{});
         p2g_kernel_launch<true>(ctx);
         mode |= kModeMuslFinish;
     }
-    k_g2p_stress<false><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
-        ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
-        (int)ctx->np, mode, BatchTables{});
+    if (mode == 0) {
+        k_g2p_stress<false, 0><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+            ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+            (int)ctx->np, 0, BatchTables{});
+    } else {
+        k_g2p_stress<false, -1><<<dim3((unsigned)nb), dim3(kG2PThreads), 0, ctx->stream>>>(
+            ctx->dP, ctx->nodes, ctx->cnt[nxt], ctx->bkt[nxt], ctx->g, ctx->mt, ctx->st,
+            (int)ctx->np, mode, BatchTables{});
+    }
     HIP_TRY(hipGetLastError());
     ctx->cur = nxt;
     if (ctx->nonlocal) {
@@ -3099,7 +3159,7 @@ static int batch_launch_g2p(mpm_hip_batch *b, int cur)
     const int nxt = 1 - cur;
     const GridParams g0{};
     const Material m0{};
-    k_g2p_stress<true><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
+    k_g2p_stress<true, 0><<<dim3(b->g2p_blocks), dim3(kG2PThreads), 0, b->stream>>>(
         b->dP, b->nodes, b->cnt[nxt], b->bkt[nxt], g0, m0, nullptr, 0, 0, batch_tables(b));
     HIP_TRY(hipGetLastError());
     return MPM_HIP_OK;
