@@ -219,6 +219,19 @@ def test_full_size_silo_five_steps():
     compare({k: sim.node(k) for k in NFIELDS}, o.nodes(), NFIELDS, "silo 1M nodes")
 
 
+def test_full_size_silo_100_steps_bitwise():
+    """The headline workload itself (512 x 512 cells, 1,048,576 particles)
+    over 100 steps, every particle field bitwise against the oracle on all
+    host threads (with the P2G dispatch order the full-size run uses)."""
+    w = workloads.silo()
+    threads = min(16, os.cpu_count() or 1)
+    sim, o = run_both(w, 100, threads=threads, flags=0)
+    compare({k: sim.get(k) for k in PFIELDS}, o.particles(), PFIELDS, "silo 1M, 100 steps",
+            tol=REL_TOL_100)
+    compare({k: sim.node(k) for k in NFIELDS}, o.nodes(), NFIELDS, "silo 1M nodes, 100 steps")
+    assert np.array_equal(sim.active(), o.particles()["active"])
+
+
 def test_full_size_silo_properties_after_200_steps():
     """Size-independent properties at full size over more steps: mass is
     conserved, a rerun is bitwise identical, nothing leaves the silo yet."""
