@@ -91,7 +91,7 @@ def cpu_baseline(w, budget_s=15.0):
 def bench_heap(args):
     """The nonlocal fluidity heap (g_nonlocal, 256x256 cells, 262,144
     particles): the same step plus the per-step fluidity assembly, a
-    Jacobi-preconditioned conjugate-gradient solve to |r| <= 1e-10 |f| and the
+    preconditioned conjugate-gradient solve to |r| <= 1e-10 |f| and the
     stress update.  The solve works on a system that stays in the last-level
     cache, so it is timed and reported, not priced against HBM."""
     w = workloads.heap()
