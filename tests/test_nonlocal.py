@@ -164,3 +164,35 @@ def test_gpu_nonlocal_heap_matches_oracle(steps, tol, blocks):
         print(f"  {g}: rel err {err:.3e} (bar {bar:g})", flush=True)
         assert err <= bar, (g, err)
     assert np.array_equal(sim.active(), r["active"])
+
+
+@pytest.mark.gpu
+def test_gpu_nonlocal_block_crossing_tiles():
+    # A compressed block sliding across the solve's 32-node tile boundary on
+    # a 64-cell grid (3 x 3 tiles): the tile it leaves turns from busy to
+    # decoupled during the run, the one it enters from decoupled to busy.
+    # The vectors a tile held while busy must not reach its neighbours'
+    # halos once it is decoupled; the run is checked against the oracle.
+    n = 64
+    h, dt = 1.0 / n, 1e-6
+    parts = workloads.block(4, 4, n, seed=12, cell_x0=26, cell_y0=6)
+    parts["sxx"][:] = -50.0
+    parts["syy"][:] = -250.0
+    parts["vx"][:] = 0.4 * h / dt
+    w = dict(name="crossing_nonlocal", N=n + 1, h=h, dt=dt, steps=30,
+             material=workloads.REFERENCE_MATERIAL,
+             nonlocal_material=workloads.REFERENCE_NONLOCAL, g_mag=0.0, ramp_time=0.1,
+             bc_kind=0, bc_fp64=(), bc_int=(), particles=parts)
+    print("case nonlocal block crossing tiles, 30 steps", flush=True)
+    sim = _gpu_nonlocal(w, w["steps"])
+    x = sim.get("x")
+    assert x.min() > 32 * h, "the block should have left the first tile column"
+    o = pyoracle.from_workload(w, num_threads=4)
+    o.run(w["steps"])
+    r = o.particles()
+    for g, k in {"x": "x", "y": "y", "vx": "vx", "vy": "vy", "txx": "sxx", "txy": "sxy",
+                 "tyy": "syy", "gf": "gf"}.items():
+        err = _rel(sim.get(g), r[k])
+        bar = 1e-9 if g == "gf" else 1e-10
+        print(f"  {g}: rel err {err:.3e} (bar {bar:g})", flush=True)
+        assert err <= bar, (g, err)
