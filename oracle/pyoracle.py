@@ -87,6 +87,8 @@ def load(variant="parity"):
     lib.oracle_band_cholesky_solve.argtypes = [D, ctypes.c_size_t, ctypes.c_size_t, D]
     lib.oracle_band_cholesky_solve.restype = ctypes.c_int
     lib.oracle_negative_root.argtypes = [ctypes.c_double] * 3
+    lib.oracle_nonlocal_bulk_point.argtypes = [ctypes.c_double] * 3 + [ctypes.POINTER(Params), D]
+    lib.oracle_nonlocal_stress_point.argtypes = [D, D] + [ctypes.c_double] * 7 + [ctypes.POINTER(Params)]
     lib.oracle_negative_root.restype = ctypes.c_double
     _libs[variant] = lib
     return lib

@@ -520,30 +520,43 @@ static void isolin_stress_threaded(threadtask_t *task)
 /* state[10] xisq_inv (g_nonlocal.c:49-59)                                  */
 /* ------------------------------------------------------------------------ */
 
-/* calculate_bulk_granular_fluidity (g_nonlocal.c:306-352): every particle. */
+/* calculate_bulk_granular_fluidity (g_nonlocal.c:306-352) for one particle:
+ * from the in-plane stress at the start of the step, out[0] = eta
+ * (state[4]), out[1] = gf_bulk (state[5]), out[2] = xisq_inv (state[10]). */
+void oracle_nonlocal_bulk_point(double sxx, double sxy, double syy,
+    const oracle_params_t *q, double *out)
+{
+    const double p_cap = q->nl_p_cap;
+    double p_t = -0.5 * (sxx + syy);
+    double sxx_tr0 = sxx + p_t;
+    double sxy_tr0 = sxy;
+    double syy_tr0 = syy + p_t;
+    double tau_t = sqrt(0.5 * (sxx_tr0 * sxx_tr0 + 2 * sxy_tr0 * sxy_tr0 + syy_tr0 * syy_tr0));
+    if (p_t < p_cap) {
+        p_t = p_cap;
+    }
+    double S0 = p_t * q->nl_mu_s;
+    double eta = q->nl_B * q->nl_d * sqrt(p_t * q->nl_rho);
+    out[0] = eta;
+    out[2] = fabs(tau_t / p_t - q->nl_mu_s) / (q->nl_d * q->nl_d * q->nl_A * q->nl_A);
+    if (tau_t > S0) {
+        out[1] = (p_t / eta) * (1 - S0 / tau_t);
+    } else {
+        out[1] = 0;
+    }
+}
+
+/* The same for every particle. */
 static void g_nonlocal_bulk(oracle_job_t *oj)
 {
     job_t *job = &oj->job;
-    const oracle_params_t *q = &oj->prm;
-    const double p_cap = q->nl_p_cap;
     for (size_t i = 0; i < job->num_particles; i++) {
         particle_t *pp = &job->particles[i];
-        double p_t = -0.5 * (pp->sxx + pp->syy);
-        double sxx_tr0 = pp->sxx + p_t;
-        double sxy_tr0 = pp->sxy;
-        double syy_tr0 = pp->syy + p_t;
-        double tau_t = sqrt(0.5 * (sxx_tr0 * sxx_tr0 + 2 * sxy_tr0 * sxy_tr0 + syy_tr0 * syy_tr0));
-        if (p_t < p_cap) {
-            p_t = p_cap;
-        }
-        double S0 = p_t * q->nl_mu_s;
-        pp->state[4] = q->nl_B * q->nl_d * sqrt(p_t * q->nl_rho);
-        pp->state[10] = fabs(tau_t / p_t - q->nl_mu_s) / (q->nl_d * q->nl_d * q->nl_A * q->nl_A);
-        if (tau_t > S0) {
-            pp->state[5] = (p_t / pp->state[4]) * (1 - S0 / tau_t);
-        } else {
-            pp->state[5] = 0;
-        }
+        double out[3];
+        oracle_nonlocal_bulk_point(pp->sxx, pp->sxy, pp->syy, &oj->prm, out);
+        pp->state[4] = out[0];
+        pp->state[5] = out[1];
+        pp->state[10] = out[2];
     }
 }
 
@@ -706,78 +719,92 @@ static int g_nonlocal_solve(oracle_job_t *oj)
     return ok ? 0 : -1;
 }
 
+/* The stress update of calculate_stress (g_nonlocal.c:148-298) for one
+ * particle: s[3] = {sxx, sxy, syy} in and out, *gammap (state[9]) advanced;
+ * gf is the particle's nonlocal fluidity (state[3]), t the job's time. */
+void oracle_nonlocal_stress_point(double *s, double *gammap, double exx_t,
+    double exy_t, double eyy_t, double wxy_t, double gf, double t, double dt,
+    const oracle_params_t *q)
+{
+    const double EMOD = q->nl_E, NUMOD = q->nl_nu, MU_S = q->nl_mu_s;
+    const double c = q->nl_c;
+    double sxx = s[0], sxy = s[1], syy = s[2];
+    double dsjxx = dt * (EMOD / (1 - NUMOD * NUMOD)) * ((exx_t) + NUMOD * (eyy_t));
+    double dsjxy = dt * (EMOD / (2 * (1 + NUMOD))) * (exy_t);
+    double dsjyy = dt * (EMOD / (1 - NUMOD * NUMOD)) * ((eyy_t) + NUMOD * (exx_t));
+    dsjxx -= 2 * dt * wxy_t * sxy;
+    dsjxy += dt * wxy_t * (sxx - syy);
+    dsjyy += 2 * dt * wxy_t * sxy;
+    double sxx_tr = sxx + dsjxx;
+    double syy_tr = syy + dsjyy;
+    double p_tr = -0.5 * (sxx_tr + syy_tr);
+    if (p_tr < c) {
+        s[0] = 0.5 * c / MU_S;
+        s[1] = 0;
+        s[2] = 0.5 * c / MU_S;
+        return;
+    }
+    /* explicit constitutive update (g_nonlocal.c:232-296) */
+    double p_t = -0.5 * (sxx + syy);
+    double sxx_t0 = sxx + p_t;
+    double sxy_t0 = sxy;
+    double syy_t0 = syy + p_t;
+    double tau_t = sqrt(0.5 * (sxx_t0 * sxx_t0 + 2 * sxy_t0 * sxy_t0 + syy_t0 * syy_t0));
+    double Npxx = sqrt(0.5) * sxx_t0 / tau_t;
+    double Npxy = sqrt(0.5) * sxy_t0 / tau_t;
+    double Npyy = sqrt(0.5) * syy_t0 / tau_t;
+    double dxx0 = 0.5 * (exx_t - eyy_t);
+    double dxy0 = exy_t;
+    double dyy0 = 0.5 * (eyy_t - exx_t);
+    double gammadot = sqrt(2 * (dxx0 * dxx0 + 2 * dxy0 * dxy0 + dyy0 * dyy0));
+    double nup_tau, dpxx, dpxy, dpyy;
+    if (p_t > c && gf > 0) {
+        nup_tau = gf * tau_t / p_t;
+        if (nup_tau < 0) nup_tau = 0;
+        if (nup_tau > gammadot) nup_tau = gammadot;
+        dpxx = sqrt(0.5) * (nup_tau) * Npxx;
+        dpxy = sqrt(0.5) * (nup_tau) * Npxy;
+        dpyy = sqrt(0.5) * (nup_tau) * Npyy;
+    } else {
+        nup_tau = 0;
+        dpxx = 0;
+        dpxy = 0;
+        dpyy = 0;
+    }
+    if (t == 0) {
+        nup_tau = 0;
+        dpxx = 0;
+        dpxy = 0;
+        dpyy = 0;
+    }
+    dsjxx = dt * (EMOD / (1 - NUMOD * NUMOD)) * ((exx_t - dpxx) + NUMOD * (eyy_t - dpyy));
+    dsjxy = dt * (EMOD / (2 * (1 + NUMOD))) * (exy_t - dpxy);
+    dsjyy = dt * (EMOD / (1 - NUMOD * NUMOD)) * ((eyy_t - dpyy) + NUMOD * (exx_t - dpxx));
+    dsjxx -= 2 * dt * wxy_t * sxy;
+    dsjxy += dt * wxy_t * (sxx - syy);
+    dsjyy += 2 * dt * wxy_t * sxy;
+    *gammap += nup_tau * dt;
+    s[0] = sxx + dsjxx;
+    s[1] = sxy + dsjxy;
+    s[2] = syy + dsjyy;
+}
+
 /* calculate_stress (g_nonlocal.c:103-301): every particle. */
 static void g_nonlocal_stress(oracle_job_t *oj)
 {
     job_t *job = &oj->job;
-    const oracle_params_t *q = &oj->prm;
-    const double EMOD = q->nl_E, NUMOD = q->nl_nu, MU_S = q->nl_mu_s;
-    const double c = q->nl_c;
     g_nonlocal_bulk(oj);
     if (g_nonlocal_solve(oj) != 0) {
         fprintf(stderr, "oracle: nonlocal solve failed\n");
     }
     for (size_t i = 0; i < job->num_particles; i++) {
         particle_t *pp = &job->particles[i];
-        const double gf = pp->state[3];
-        double dsjxx = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->exx_t) + NUMOD * (pp->eyy_t));
-        double dsjxy = job->dt * (EMOD / (2 * (1 + NUMOD))) * (pp->exy_t);
-        double dsjyy = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->eyy_t) + NUMOD * (pp->exx_t));
-        dsjxx -= 2 * job->dt * pp->wxy_t * pp->sxy;
-        dsjxy += job->dt * pp->wxy_t * (pp->sxx - pp->syy);
-        dsjyy += 2 * job->dt * pp->wxy_t * pp->sxy;
-        double sxx_tr = pp->sxx + dsjxx;
-        double syy_tr = pp->syy + dsjyy;
-        double p_tr = -0.5 * (sxx_tr + syy_tr);
-        if (p_tr < c) {
-            pp->sxx = 0.5 * c / MU_S;
-            pp->sxy = 0;
-            pp->syy = 0.5 * c / MU_S;
-            continue;
-        }
-        /* explicit constitutive update (g_nonlocal.c:232-296) */
-        double p_t = -0.5 * (pp->sxx + pp->syy);
-        double sxx_t0 = pp->sxx + p_t;
-        double sxy_t0 = pp->sxy;
-        double syy_t0 = pp->syy + p_t;
-        double tau_t = sqrt(0.5 * (sxx_t0 * sxx_t0 + 2 * sxy_t0 * sxy_t0 + syy_t0 * syy_t0));
-        double Npxx = sqrt(0.5) * sxx_t0 / tau_t;
-        double Npxy = sqrt(0.5) * sxy_t0 / tau_t;
-        double Npyy = sqrt(0.5) * syy_t0 / tau_t;
-        double dxx0 = 0.5 * (pp->exx_t - pp->eyy_t);
-        double dxy0 = pp->exy_t;
-        double dyy0 = 0.5 * (pp->eyy_t - pp->exx_t);
-        double gammadot = sqrt(2 * (dxx0 * dxx0 + 2 * dxy0 * dxy0 + dyy0 * dyy0));
-        double nup_tau, dpxx, dpxy, dpyy;
-        if (p_t > c && gf > 0) {
-            nup_tau = gf * tau_t / p_t;
-            if (nup_tau < 0) nup_tau = 0;
-            if (nup_tau > gammadot) nup_tau = gammadot;
-            dpxx = sqrt(0.5) * (nup_tau) * Npxx;
-            dpxy = sqrt(0.5) * (nup_tau) * Npxy;
-            dpyy = sqrt(0.5) * (nup_tau) * Npyy;
-        } else {
-            nup_tau = 0;
-            dpxx = 0;
-            dpxy = 0;
-            dpyy = 0;
-        }
-        if (job->t == 0) {
-            nup_tau = 0;
-            dpxx = 0;
-            dpxy = 0;
-            dpyy = 0;
-        }
-        dsjxx = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->exx_t - dpxx) + NUMOD * (pp->eyy_t - dpyy));
-        dsjxy = job->dt * (EMOD / (2 * (1 + NUMOD))) * (pp->exy_t - dpxy);
-        dsjyy = job->dt * (EMOD / (1 - NUMOD * NUMOD)) * ((pp->eyy_t - dpyy) + NUMOD * (pp->exx_t - dpxx));
-        dsjxx -= 2 * job->dt * pp->wxy_t * pp->sxy;
-        dsjxy += job->dt * pp->wxy_t * (pp->sxx - pp->syy);
-        dsjyy += 2 * job->dt * pp->wxy_t * pp->sxy;
-        pp->state[9] += nup_tau * job->dt;
-        pp->sxx += dsjxx;
-        pp->sxy += dsjxy;
-        pp->syy += dsjyy;
+        double s[3] = {pp->sxx, pp->sxy, pp->syy};
+        oracle_nonlocal_stress_point(s, &pp->state[9], pp->exx_t, pp->exy_t,
+            pp->eyy_t, pp->wxy_t, pp->state[3], job->t, job->dt, &oj->prm);
+        pp->sxx = s[0];
+        pp->sxy = s[1];
+        pp->syy = s[2];
     }
 }
 
