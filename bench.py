@@ -6,15 +6,23 @@ walls, dt = 2e-6.  A step is one explicit_mpm_step_usl_threaded +
 time_varying_loads over all particles, with the state resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+                    [--no-subrecords] [--workload silo|heap|ensemble|adjoint]
 
-Prints one JSON line.  `value` is particle-steps per second on the GPU.
+Prints one JSON line.  `value` is particle-steps per second on the GPU over
+K steps timed between two HIP events (nothing else recorded in between);
+`ms_per_step_wall` is the host clock around the same call, a cross-check.
 `roofline` prices the dominant kernel's algorithmic bytes (FP64 SoA, each
 field moved once per pass that needs it) against the 8 TB/s HBM peak, with
-kernel durations from HIP events recorded on the library's stream during the
-timed steps.  `cpu_baseline` times the CPU restatement of the reference step
-(reference AoS layout, pthreads, colour lists, barriers; compiled with the
-reference's -O3 and default FP contraction) on the host's cores over a
-bounded sample of the same workload.
+its duration from HIP events recorded around every kernel in a second pass
+of K steps.  The CPU-baseline leg first gates accuracy (GPU against the
+oracle's parity build after 1 and 100 steps) and then times the CPU
+restatement of the reference step (reference AoS layout, pthreads, colour
+lists, barriers; compiled with the reference's -O3 and default FP
+contraction) on the host's cores over a bounded sample of the same
+workload.  The sub-records `batched` and `adjoint` (with
+`batched_vs_plain` and `adjoint_vs_plain`) measure the 256-member ensemble
+against the same members as single jobs and the 200-step adjoint against
+its plain forward run, each with its own accuracy check.
 """
 import argparse
 import json
@@ -143,7 +151,7 @@ def bench_heap(args):
     print(json.dumps(result), flush=True)
 
 
-def bench_ensemble(args):
+def bench_ensemble(args, emit=True):
     """256 independent column collapses of uneven size (33^2 to 181^2
     nodes, 2,048 to 64,800 particles, 4,661,824 in all; per-member mu_s and
     E) stepped together by shared launches replayed from a HIP graph, after
@@ -167,6 +175,8 @@ def bench_ensemble(args):
     ms = b.time_steps(args.steps)
     wall = time.perf_counter() - t0
     nact1 = sum(int(b.active(k).sum()) for k in range(len(ens)))
+    acc_fields = ("x", "y", "vx", "vy", "txx", "txy", "tyy")
+    batched_state = [{f: b.get(k, f) for f in acc_fields} for k in range(len(ens))]
     b.close()
     pstep = (nact0 + nact1) / 2.0 * args.steps
     np_sum, nn_sum = sum(counts), sum(nodes)
@@ -179,8 +189,9 @@ def bench_ensemble(args):
     step_bytes = p2g_bytes + g2p_bytes
     step_ms = ms[0] / args.steps
 
-    # the same members as single-job contexts, stepped round-robin
-    plain_steps = max(2, min(args.steps, 50))
+    # the same members as single-job contexts, stepped round-robin for as
+    # many steps as the batch took, so that the end states must agree
+    plain_steps = args.steps
     sims = [pympm.Sim(c, w["particles"]) for c, w in zip(cfgs, ens)]
     for s in sims:
         s.step(args.warmup)
@@ -194,6 +205,8 @@ def bench_ensemble(args):
         s.sync()
     plain_wall = time.perf_counter() - t1
     plain_act = sum(int(s.active().sum()) for s in sims)
+    mismatched = [k for k, s in enumerate(sims)
+                  if not all(np.array_equal(s.get(f), batched_state[k][f]) for f in acc_fields)]
     for s in sims:
         s.close()
     plain_value = plain_act * plain_steps / plain_wall
@@ -224,21 +237,33 @@ def bench_ensemble(args):
         "plain": {"value": plain_value, "steps": plain_steps,
                   "how": "256 single-job contexts, one stream each, stepped round-robin"},
         "batched_vs_plain": (pstep / (ms[0] * 1e-3)) / plain_value,
+        "accuracy": {"checked": f"every member after {args.warmup + args.steps} steps, fields "
+                                f"{list(acc_fields)}, against the single-job contexts",
+                     "members_bitwise_equal": len(ens) - len(mismatched),
+                     "members": len(ens), "passed": not mismatched},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle restatement on a sample of members)")
         result["cpu_baseline"] = cpu_baseline_members(ens[::32], args.cpu_budget)
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
-    print(json.dumps(result), flush=True)
+    if emit:
+        print(json.dumps(result), flush=True)
+    return result
 
 
-def bench_adjoint(args):
+def bench_adjoint(args, emit=True):
     """The adjoint workload (129^2 nodes, 65,536 particles, 200 steps,
     checkpoint every 20): a forward run keeping checkpoints, then the reverse
-    sweep for L = |x - x*|^2 + |y - y*|^2 against a run with the target
+    sweep of L = (front - front*)^2 + (cx - cx*)^2 + (cy - cy*)^2 (runout
+    front and particle centroid, calibrate.py) against a run with the target
     constants (mu_s = 0.42, E = 2e6).  A particle-step is one forward step
-    and its reverse; the plain forward run is timed beside it."""
+    and its reverse.  Beside it, the plain forward run (mpm_hip_step, device
+    time of the 200 steps).  Accuracy: the reverse sweep's dL/dmu_s, dL/dE
+    and dL/d(initial velocities) along a seeded direction against central
+    differences of the forward run (relative step 1e-6 for the constants,
+    1e-6 m/s along the direction; bound 1e-5)."""
+    import calibrate
     w = workloads.adjoint_block()
     steps = w["steps"]
     npart = len(w["particles"]["x"])
@@ -250,18 +275,16 @@ def bench_adjoint(args):
                                  ramp_time=w["ramp_time"])
     tsim = pympm.Sim(cfg_for(workloads.ADJOINT_TARGET), w["particles"])
     tsim.step(steps)
-    tx, ty = tsim.get("x"), tsim.get("y")
+    target, _ = calibrate.observables(tsim.get("x"), tsim.get("y"), tsim.active())
     tsim.close()
+    warm = pympm.Sim(cfg_for(w["material"]), w["particles"])
+    warm.step(steps)
+    warm.sync()
+    warm.close()
     plain = pympm.Sim(cfg_for(w["material"]), w["particles"])
-    plain.step(steps)      # warm-up
-    plain.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.repeats):
-        plain.set("x", w["particles"]["x"])
-        plain.step(steps)
-        plain.sync()
-    t_plain = (time.perf_counter() - t0) / args.repeats
+    t_plain = plain.time_span(steps) * 1e-3     # the 200 steps from the initial state
     plain.close()
+
     adj = pympm.Adjoint(cfg_for(w["material"]), w["particles"], checkpoint_every=20)
     adj.forward(steps)
     adj.backward({"x": np.zeros(npart)})    # warm-up
@@ -270,17 +293,44 @@ def bench_adjoint(args):
         t0 = time.perf_counter()
         adj.forward(steps)
         t1 = time.perf_counter()
-        x, y = adj.get("x"), adj.get("y")
+        loss, seed = calibrate.loss_and_seed(adj.get("x"), adj.get("y"), adj.active(), target)
         t2 = time.perf_counter()
-        grad, _ = adj.backward({"x": 2 * (x - tx), "y": 2 * (y - ty)})
+        grad, gi = adj.backward(seed)
         t3 = time.perf_counter()
         t_f += t1 - t0
         t_b += t3 - t2
     t_f /= args.repeats
     t_b /= args.repeats
-    adj.close()
-    loss = float(np.sum((x - tx) ** 2) + np.sum((y - ty) ** 2))
     total = t_f + t_b
+
+    # central differences of the forward run
+    def loss_at(mat=None, vx=None, vy=None):
+        if mat is not None:
+            adj.set_material(**mat)
+        if vx is not None:
+            adj.set_particles(dict(w["particles"], vx=vx, vy=vy))
+        adj.forward(steps)
+        val, _ = calibrate.loss_and_seed(adj.get("x"), adj.get("y"), adj.active(), target)
+        return val
+    base = dict(w["material"])
+    fd = {}
+    for k in ("mu_s", "E"):
+        hk = 1e-6 * abs(base[k])
+        lp = loss_at(dict(base, **{k: base[k] + hk}))
+        lm = loss_at(dict(base, **{k: base[k] - hk}))
+        fd[k] = ((lp - lm) / (2 * hk), grad[k])
+    adj.set_material(**base)
+    rng = np.random.default_rng(7)
+    ux, uy = rng.uniform(-1, 1, npart), rng.uniform(-1, 1, npart)
+    hv = 1e-6
+    p0 = w["particles"]
+    lp = loss_at(vx=p0["vx"] + hv * ux, vy=p0["vy"] + hv * uy)
+    lm = loss_at(vx=p0["vx"] - hv * ux, vy=p0["vy"] - hv * uy)
+    fd["initial velocity (direction)"] = ((lp - lm) / (2 * hv),
+                                          float(np.dot(ux, gi["vx"]) + np.dot(uy, gi["vy"])))
+    adj.close()
+    rel = {k: abs(a - b) / max(abs(a), 1e-300) for k, (a, b) in fd.items()}
+
     result = {
         "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
         "value": npart * steps / total,
@@ -291,11 +341,15 @@ def bench_adjoint(args):
         "data": "synthetic (seeded lattice + jitter, lithostatic stress, initial slide)",
         "config": {"workload": "adjoint block, 128x128 cells (129^2 nodes), 65,536 particles, "
                                "200 steps, checkpoint every 20, g_local_mu2 "
-                               "(mu_s 0.38 vs target 0.42, E 1e6 vs target 2e6)",
+                               "(mu_s 0.38 vs target 0.42, E 1e6 vs target 2e6), loss = "
+                               "squared error of the final runout front and centroid",
                    "particles": npart, "nodes": w["N"] ** 2, "parallelism": "single GPU"},
         "forward_ms": t_f * 1e3, "reverse_ms": t_b * 1e3, "plain_forward_ms": t_plain * 1e3,
         "adjoint_vs_plain": t_plain / total,
         "loss": loss, "dL_dmu_s": grad["mu_s"], "dL_dE": grad["E"],
+        "accuracy": {"central_differences": {k: {"fd": a, "adjoint": b, "rel_err": rel[k]}
+                                             for k, (a, b) in fd.items()},
+                     "bound": 1e-5, "passed": all(v <= 1e-5 for v in rel.values())},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -305,8 +359,8 @@ def bench_adjoint(args):
         nsteps = 20
         t0 = time.perf_counter()
         recs, fin = adjoint_oracle.record(w, nsteps, num_threads=1)
-        adjoint_oracle.adjoint(recs, {"x": fin["x"], "y": fin["y"]}, w["material"], w["N"],
-                               w["h"], w["dt"])
+        _, seed20 = calibrate.loss_and_seed(fin["x"], fin["y"], fin["active"], target)
+        adjoint_oracle.adjoint(recs, seed20, w["material"], w["N"], w["h"], w["dt"])
         el = time.perf_counter() - t0
         result["cpu_baseline"] = {
             "value": npart * nsteps / el, "unit": "particle-steps/s (forward + reverse)",
@@ -315,7 +369,9 @@ def bench_adjoint(args):
                        "forward on 1 thread, recorded step by step, then the numpy reverse "
                        "sweep of oracle/adjoint_oracle.py")}
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
-    print(json.dumps(result), flush=True)
+    if emit:
+        print(json.dumps(result), flush=True)
+    return result
 
 
 def cpu_baseline_members(sample, budget_s):
@@ -374,7 +430,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU-baseline leg (and with it the accuracy gate)")
+    ap.add_argument("--no-subrecords", action="store_true",
+                    help="silo only: no batched-ensemble and adjoint sub-records")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--repeats", type=int, default=3, help="adjoint workload: timed runs")
     ap.add_argument("--workload", choices=["silo", "heap", "ensemble", "adjoint"],
@@ -402,6 +461,32 @@ def main():
     if args.workload == "adjoint":
         bench_adjoint(args)
         return
+    result = bench_silo(args)
+    if not args.no_subrecords:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.no_cpu_baseline, sub.repeats = 20, 5, True, 1
+        log("bench: sub-record: batched ensemble against the same members as plain steps")
+        ens = bench_ensemble(sub, emit=False)
+        result["batched_vs_plain"] = ens["batched_vs_plain"]
+        result["batched"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "config",
+                                                 "roofline", "plain", "accuracy")}
+        log("bench: sub-record: adjoint (forward + reverse) against the plain forward run")
+        adj = bench_adjoint(sub, emit=False)
+        result["adjoint_vs_plain"] = adj["adjoint_vs_plain"]
+        result["adjoint"] = {k: adj[k] for k in ("value", "unit", "forward_ms", "reverse_ms",
+                                                 "plain_forward_ms", "config", "accuracy")}
+    print(json.dumps(result), flush=True)
+
+
+def bench_silo(args):
+    """The headline: the square silo, state resident in HBM.  Pass A times
+    `steps` steps between two HIP events and nothing else (value,
+    ms_per_step); pass B runs the same number of steps with events around
+    every kernel, for the per-kernel durations the roofline uses.  Wall time
+    around pass A is kept as a cross-check.  In the CPU-baseline leg the
+    accuracy gate compares a separate GPU run of the same workload with the
+    oracle's parity build after 1 and after 100 steps (bounds 1e-12 and 1e-8)
+    before the CPU restatement is timed."""
     w = workloads.silo()
     npart = len(w["particles"]["x"])
     nnodes = w["N"] * w["N"]
@@ -416,11 +501,13 @@ def main():
     nact0 = int(sim.active().sum())
 
     t0 = time.perf_counter()
-    ms = sim.time_steps(args.steps)       # kernels bracketed by HIP events
-    sim.sync()
+    span_ms = sim.time_span(args.steps)       # pass A: two events, n steps
     wall = time.perf_counter() - t0
     nact1 = int(sim.active().sum())
+    ms = sim.time_steps(args.steps)           # pass B: events around every kernel
+    sim.close()
     pstep = (nact0 + nact1) / 2.0 * args.steps
+    step_ms = span_ms / args.steps
 
     p2g_ms, g2p_ms = ms[1] / args.steps, ms[2] / args.steps
     p2g_bytes = P2G_BYTES_PER_PARTICLE * npart + P2G_BYTES_PER_NODE * nnodes
@@ -435,12 +522,14 @@ def main():
     step_bytes = p2g_bytes + g2p_bytes
     result = {
         "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
-        "value": pstep / wall,
+        "value": pstep / (span_ms * 1e-3),
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": step_ms,
+        "ms_per_step_wall": wall * 1e3 / args.steps,
+        "ms_per_step_kernel_sum": (ms[1] + ms[2]) / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -455,16 +544,55 @@ def main():
                      "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
         "kernels_ms": {k: v[0] for k, v in kernels.items()},
         "step_roofline": {"algorithmic_bytes": step_bytes,
-                          "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
-                          "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+                          "achieved_GBs": step_bytes / (step_ms * 1e-3) / 1e9,
+                          "frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "accuracy": None,
         "cpu_baseline": None,
     }
-    sim.close()
     if not args.no_cpu_baseline:
-        log("bench: CPU baseline (oracle restatement, all host cores)")
+        log("bench: CPU-baseline leg: accuracy gate against the oracle's parity build")
+        result["accuracy"] = accuracy_gate(w, cfg)
+        if not result["accuracy"]["passed"]:
+            print(json.dumps(result), flush=True)
+            sys.exit("bench: the accuracy gate failed; the throughput does not count")
+        log("bench: CPU-baseline leg: oracle restatement on all host cores")
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
-    print(json.dumps(result), flush=True)
+    return result
+
+
+GATE_FIELDS = ("x", "y", "vx", "vy", "txx", "txy", "tyy")
+
+
+def accuracy_gate(w, cfg, steps=(1, 100)):
+    """Per-field relative error max|gpu - cpu| / max|cpu| of position,
+    velocity and stress after steps[0] (bound 1e-12) and steps[1] (bound
+    1e-8), the GPU run against the oracle's parity build (-ffp-contract=off,
+    portable exp) on identical inputs.  Part of the CPU-baseline leg."""
+    import pyoracle
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    sim = pympm.Sim(cfg, w["particles"])
+    o = pyoracle.from_workload(w, num_threads=threads)
+    done, out = 0, {"steps": list(steps), "bounds": [1e-12, 1e-8], "max_rel_err": [],
+                    "bitwise": [], "fields": list(GATE_FIELDS), "checker": "oracle parity build",
+                    "threads": threads}
+    for n in steps:
+        sim.step(n - done)
+        o.run(n - done)
+        done = n
+        ref = o.particles()
+        err, same = 0.0, True
+        for k in GATE_FIELDS:
+            got = sim.get(k)
+            scale = float(np.abs(ref[k]).max()) or 1.0
+            err = max(err, float(np.abs(got - ref[k]).max()) / scale)
+            same = same and bool(np.array_equal(got, ref[k]))
+        out["max_rel_err"].append(err)
+        out["bitwise"].append(same)
+    sim.close()
+    o.close()
+    out["passed"] = all(e <= b for e, b in zip(out["max_rel_err"], out["bounds"]))
+    return out
 
 
 if __name__ == "__main__":

@@ -237,6 +237,11 @@ int mpm_hip_get_solver_stats(mpm_hip_ctx *ctx, int *iterations, double *rel_resi
  * stream.  ms[0] = whole span, ms[1] = sum of P2G+grid kernel times,
  * ms[2] = sum of G2P+stress kernel times (per-kernel events). */
 int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms);
+/* Runs n steps between two HIP events on the context's stream (created
+ * before the first is recorded) and no others: ms[0] = the span, the
+ * steady-state step time the per-kernel events of mpm_hip_time_steps would
+ * perturb.  The timed stretch of mpm_run_until (src/main.c:828-844). */
+int mpm_hip_time_span(mpm_hip_ctx *ctx, int n, double *ms);
 
 int mpm_hip_destroy(mpm_hip_ctx *ctx);
 

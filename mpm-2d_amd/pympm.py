@@ -94,6 +94,7 @@ def load():
     lib.mpm_hip_get_time.argtypes = [P, D, D]
     lib.mpm_hip_get_crossings.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     lib.mpm_hip_time_steps.argtypes = [P, I, D]
+    lib.mpm_hip_time_span.argtypes = [P, I, D]
     lib.mpm_hip_get_solver_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_destroy.argtypes = [P]
     lib.mpm_hip_status_string.argtypes = [I]
@@ -245,6 +246,12 @@ class Sim:
         ms = np.zeros(3, dtype=np.float64)
         _check(self.lib.mpm_hip_time_steps(self.ctx, n, _dptr(ms)), "mpm_hip_time_steps")
         return ms
+
+    def time_span(self, n):
+        """Device time (ms) of n steps between two events, no others."""
+        ms = np.zeros(1, dtype=np.float64)
+        _check(self.lib.mpm_hip_time_span(self.ctx, n, _dptr(ms)), "mpm_hip_time_span")
+        return float(ms[0])
 
     def state(self, names=None):
         names = names or FIELDS[:14]
