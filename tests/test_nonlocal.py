@@ -196,3 +196,36 @@ def test_gpu_nonlocal_block_crossing_tiles():
         bar = 1e-9 if g == "gf" else 1e-10
         print(f"  {g}: rel err {err:.3e} (bar {bar:g})", flush=True)
         assert err <= bar, (g, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cells,blocks", [(16, 0), (64, 1)])
+def test_gpu_zero_fluidity_load_after_warm_start(cells, blocks):
+    """A block above yield (tau / p = 0.67) for two steps, so the solve's
+    warm start x is nonzero; then its stress is made isotropic: tau = 0
+    everywhere, the bulk fluidity and f vanish, and the solve must return
+    x = 0 (what the reference's direct solve gives) at once instead of
+    iterating towards it against a stopping bound of zero.  (16 cells: one
+    tile per workgroup; 64 cells with one workgroup: the Jacobi path.)"""
+    n = cells
+    parts = workloads.block(6, 6, n, seed=9, cell_x0=n // 3, cell_y0=n // 3)
+    parts["sxx"][:] = -50.0
+    parts["syy"][:] = -250.0
+    w = dict(name="zero_load_nonlocal", N=n + 1, h=1 / n, dt=1e-6, steps=2,
+             material=workloads.REFERENCE_MATERIAL,
+             nonlocal_material=workloads.REFERENCE_NONLOCAL, g_mag=0.0, ramp_time=0.1,
+             bc_kind=0, bc_fp64=(), bc_int=(), particles=parts)
+    print(f"case nonlocal zero load after warm start, {cells} cells, blocks={blocks}", flush=True)
+    sim = _gpu_nonlocal(w, 2, tol=1e-10, cg_max_blocks=blocks)
+    assert np.abs(sim.get("gf")).max() > 0
+    iso = 0.5 * (sim.get("txx") + sim.get("tyy"))
+    sim.set("txx", iso)
+    sim.set("tyy", iso)
+    sim.set("txy", np.zeros_like(iso))
+    sim.step(1)
+    sim.sync()
+    it, res = sim.solver_stats()
+    assert (it, res) == (0, 0.0)
+    assert np.all(sim.get("gf_bulk") == 0.0)
+    assert np.all(sim.get("gf") == 0.0)
+    sim.close()
