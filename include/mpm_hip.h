@@ -335,6 +335,33 @@ int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
                              double *grad_params, double *const *grad_init);
 int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a);
 
+
+/* ---------------------------------------------------------------------
+ * A step that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory per call (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs n USL steps exactly as mpm_hip_create + mpm_hip_set_particles +
+ * mpm_hip_step would (bitwise the same trajectory), on particle arrays the
+ * caller keeps in device memory:
+ *   fields[f]  device arrays of num_particles doubles for f = MPM_F_X ..
+ *              MPM_F_UY (in and out; MPM_F_TXX/TXY/TYY start as the
+ *              particle file's sxx, sxy, syy, MPM_F_TZZ and the state at 0)
+ *   active     device int array (in and out; mpm_init's flags at the start)
+ *   clock      device array of 4 doubles: in {t, bx, by, -}, out {t, bx, by,
+ *              status} with status 1.0 if a cell overflowed its bin of 32
+ *              (results then invalid), else 0.0; a run starts from
+ *              {cfg->t0, 0, 0} (initial_loads)
+ *   stream     a hipStream_t, NULL for the null stream
+ * The bins, node records and the rest live in one static device array of
+ * about 6.5 MB per device, which limits a job to cfg->N <= 129 nodes per
+ * side and 262,144 particles (larger jobs are rejected); the array keeps
+ * the last call's contents, every call clears what it needs, and calls that
+ * overlap on one device race.  Returns after enqueueing (no wait); USL,
+ * bin capacity 32, no flags, no MPM_BC_MASK, not g_nonlocal. */
+int mpm_hip_step_noalloc(const mpm_hip_config *cfg, size_t num_particles,
+                         double *const *fields, int *active, double *clock, int n,
+                         void *stream);
+
 /* Test hooks: the library's exp() on the host, and device evaluation of
  * exp, division and square root over n inputs (op 0: exp(a), 1: a / b,
  * 2: sqrt(a)) to check bitwise agreement with the host. */

@@ -40,6 +40,8 @@
 //   mpm_job.inc               a single job's context, launches and C ABI
 //   mpm_batch.inc             ensembles: batch, dispatch tables, graph replay
 //   mpm_adjoint.inc           adjoint runs and the remaining C-ABI helpers
+//   mpm_noalloc.inc           the step on caller-owned arrays with a static
+//                             scratch array (no allocation per call)
 
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
@@ -50,6 +52,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 
@@ -68,3 +71,4 @@ namespace {
 #include "mpm_job.inc"
 #include "mpm_batch.inc"
 #include "mpm_adjoint.inc"
+#include "mpm_noalloc.inc"
