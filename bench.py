@@ -293,7 +293,8 @@ def bench_adjoint(args, emit=True):
         t0 = time.perf_counter()
         adj.forward(steps)
         t1 = time.perf_counter()
-        loss, seed = calibrate.loss_and_seed(adj.get("x"), adj.get("y"), adj.active(), target)
+        xf, yf, af = adj.get("x"), adj.get("y"), adj.active()
+        loss, seed = calibrate.loss_and_seed(xf, yf, af, target)
         t2 = time.perf_counter()
         grad, gi = adj.backward(seed)
         t3 = time.perf_counter()
@@ -303,33 +304,45 @@ def bench_adjoint(args, emit=True):
     t_b /= args.repeats
     total = t_f + t_b
 
-    # central differences of the forward run
-    def loss_at(mat=None, vx=None, vy=None):
+    # central differences of the forward run: the constants at relative
+    # steps 1e-5 (the bound's) and 1e-6 (reported), the initial velocities
+    # of the front particle and the largest gradient components at 1e-6 m/s
+    def loss_at(mat=None, particles=None):
         if mat is not None:
             adj.set_material(**mat)
-        if vx is not None:
-            adj.set_particles(dict(w["particles"], vx=vx, vy=vy))
+        if particles is not None:
+            adj.set_particles(particles)
         adj.forward(steps)
         val, _ = calibrate.loss_and_seed(adj.get("x"), adj.get("y"), adj.active(), target)
         return val
     base = dict(w["material"])
     fd = {}
     for k in ("mu_s", "E"):
-        hk = 1e-6 * abs(base[k])
-        lp = loss_at(dict(base, **{k: base[k] + hk}))
-        lm = loss_at(dict(base, **{k: base[k] - hk}))
-        fd[k] = ((lp - lm) / (2 * hk), grad[k])
+        for rs in (1e-5, 1e-6):
+            hk = rs * abs(base[k])
+            lp = loss_at(dict(base, **{k: base[k] + hk}))
+            lm = loss_at(dict(base, **{k: base[k] - hk}))
+            fd[f"{k} (h={rs:g}|{k}|)"] = ((lp - lm) / (2 * hk), grad[k], rs == 1e-5)
     adj.set_material(**base)
-    rng = np.random.default_rng(7)
-    ux, uy = rng.uniform(-1, 1, npart), rng.uniform(-1, 1, npart)
-    hv = 1e-6
     p0 = w["particles"]
-    lp = loss_at(vx=p0["vx"] + hv * ux, vy=p0["vy"] + hv * uy)
-    lm = loss_at(vx=p0["vx"] - hv * ux, vy=p0["vy"] - hv * uy)
-    fd["initial velocity (direction)"] = ((lp - lm) / (2 * hv),
-                                          float(np.dot(ux, gi["vx"]) + np.dot(uy, gi["vy"])))
+    front = calibrate.observables(xf, yf, af)[1]
+    picks = [("vx", front)] \
+        + [("vx", int(i)) for i in np.argsort(-np.abs(gi["vx"]))[:2]] \
+        + [("vy", int(i)) for i in np.argsort(-np.abs(gi["vy"]))[:2]]
+    for f, i in picks:
+        vals = []
+        for sgn in (1, -1):
+            p2 = {q: v.copy() for q, v in p0.items()}
+            p2[f][i] += sgn * 1e-6
+            vals.append(loss_at(particles=p2))
+        fd[f"{f}0[{i}] (h=1e-6)"] = ((vals[0] - vals[1]) / 2e-6, float(gi[f][i]), True)
+    adj.set_particles(p0)
+    rel = {k: float(abs(a - b) / max(abs(a), 1e-300)) for k, (a, b, _) in fd.items()}
+    # the calibration the gradient is for: Gauss-Newton from (0.38, 1e6)
+    t0 = time.perf_counter()
+    cal = calibrate.calibrate(adj, steps, target, base["mu_s"], base["E"])
+    cal_s = time.perf_counter() - t0
     adj.close()
-    rel = {k: abs(a - b) / max(abs(a), 1e-300) for k, (a, b) in fd.items()}
 
     result = {
         "metric": "particle-steps/sec FP64 USL step, % HBM roofline; batched+adjoint vs plain",
@@ -347,9 +360,20 @@ def bench_adjoint(args, emit=True):
         "forward_ms": t_f * 1e3, "reverse_ms": t_b * 1e3, "plain_forward_ms": t_plain * 1e3,
         "adjoint_vs_plain": t_plain / total,
         "loss": loss, "dL_dmu_s": grad["mu_s"], "dL_dE": grad["E"],
-        "accuracy": {"central_differences": {k: {"fd": a, "adjoint": b, "rel_err": rel[k]}
-                                             for k, (a, b) in fd.items()},
-                     "bound": 1e-5, "passed": all(v <= 1e-5 for v in rel.values())},
+        "accuracy": {"central_differences": {k: {"fd": float(a), "adjoint": float(b),
+                                                 "rel_err": rel[k],
+                                                 "gated": g}
+                                             for k, (a, b, g) in fd.items()},
+                     "bound": 1e-5,
+                     "passed": bool(all(rel[k] <= 1e-5 for k, (_, _, g) in fd.items() if g))},
+        "calibration": {"start": {"mu_s": base["mu_s"], "E": base["E"]},
+                        "target": {"mu_s": workloads.ADJOINT_TARGET["mu_s"],
+                                   "E": workloads.ADJOINT_TARGET["E"]},
+                        "found": {"mu_s": cal["mu_s"], "E": cal["E"]},
+                        "iterations": cal["iterations"], "residual": cal["residual"],
+                        "seconds": cal_s,
+                        "how": "Gauss-Newton on the three residuals, Jacobian from three "
+                               "reverse sweeps per iteration (calibrate.py)"},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -360,8 +384,23 @@ def bench_adjoint(args, emit=True):
         t0 = time.perf_counter()
         recs, fin = adjoint_oracle.record(w, nsteps, num_threads=1)
         _, seed20 = calibrate.loss_and_seed(fin["x"], fin["y"], fin["active"], target)
-        adjoint_oracle.adjoint(recs, seed20, w["material"], w["N"], w["h"], w["dt"])
+        lam0, gcpu = adjoint_oracle.adjoint(recs, seed20, w["material"], w["N"], w["h"],
+                                            w["dt"])
         el = time.perf_counter() - t0
+        # the GPU reverse sweep of the same 20 steps, against it
+        a20 = pympm.Adjoint(cfg_for(w["material"]), w["particles"], checkpoint_every=20)
+        a20.forward(nsteps)
+        gg, ggi = a20.backward(seed20)
+        a20.close()
+        cmp = {k: abs(gg[k] - gcpu[adjoint_oracle.PARAMS.index(k)])
+               / abs(gcpu[adjoint_oracle.PARAMS.index(k)]) for k in ("mu_s", "E")}
+        for k in ("vx", "vy"):
+            cmp[k] = float(np.abs(ggi[k] - lam0[k]).max() / np.abs(lam0[k]).max())
+        cmp = {k: float(v) for k, v in cmp.items()}
+        result["accuracy"]["cpu_adjoint_20_steps"] = {"rel_err": cmp, "bound": 1e-9,
+                                                      "passed": bool(max(cmp.values()) <= 1e-9)}
+        result["accuracy"]["passed"] = bool(result["accuracy"]["passed"]
+                                            and max(cmp.values()) <= 1e-9)
         result["cpu_baseline"] = {
             "value": npart * nsteps / el, "unit": "particle-steps/s (forward + reverse)",
             "cores": 1, "kind": "port",
@@ -471,10 +510,12 @@ def main():
         result["batched"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "config",
                                                  "roofline", "plain", "accuracy")}
         log("bench: sub-record: adjoint (forward + reverse) against the plain forward run")
+        sub.no_cpu_baseline = args.no_cpu_baseline   # its CPU leg also checks the reverse sweep
         adj = bench_adjoint(sub, emit=False)
         result["adjoint_vs_plain"] = adj["adjoint_vs_plain"]
         result["adjoint"] = {k: adj[k] for k in ("value", "unit", "forward_ms", "reverse_ms",
-                                                 "plain_forward_ms", "config", "accuracy")}
+                                                 "plain_forward_ms", "config", "accuracy",
+                                                 "calibration", "cpu_baseline")}
     print(json.dumps(result), flush=True)
 
 

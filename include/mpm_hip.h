@@ -335,6 +335,20 @@ int mpm_hip_adjoint_backward(mpm_hip_adjoint *a, const double *const *seed,
                              double *grad_params, double *const *grad_init);
 int mpm_hip_adjoint_destroy(mpm_hip_adjoint *a);
 
+/* The same three calls with device pointers (memory on the adjoint's GPU,
+ * e.g. a torch tensor's storage): the initial state is copied device to
+ * device, the fields likewise, and the seeds and dL/d(initial state) stay
+ * on the device (grad_params is still host memory: seven doubles).  Each
+ * returns after the copies have completed; the caller makes its own
+ * stream's writes to the inputs complete before the call.  For the
+ * torch.autograd bridge (mpm-2d_amd/usl_autograd.py). */
+int mpm_hip_adjoint_set_particles_device(mpm_hip_adjoint *a, const double *m, const double *v,
+                                         const double *x, const double *y, const double *vx,
+                                         const double *vy, const double *sxx, const double *sxy,
+                                         const double *syy);
+int mpm_hip_adjoint_get_field_device(mpm_hip_adjoint *a, int field, double *dst);
+int mpm_hip_adjoint_backward_device(mpm_hip_adjoint *a, const double *const *seed,
+                                    double *grad_params, double *const *grad_init);
 
 /* ---------------------------------------------------------------------
  * A step that allocates nothing.  No counterpart in the reference: it is

@@ -122,6 +122,9 @@ def load():
     lib.mpm_hip_adjoint_get_active.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     lib.mpm_hip_adjoint_backward.argtypes = [P, DP, D, DP]
     lib.mpm_hip_adjoint_destroy.argtypes = [P]
+    lib.mpm_hip_adjoint_set_particles_device.argtypes = [P] + [P] * 9
+    lib.mpm_hip_adjoint_get_field_device.argtypes = [P, I, P]
+    lib.mpm_hip_adjoint_backward_device.argtypes = [P, ctypes.POINTER(P), D, ctypes.POINTER(P)]
     _lib = lib
     return lib
 
@@ -364,12 +367,16 @@ PARAMS = ["E", "nu", "mu_s", "mu_2", "I_0", "grains_d", "grains_rho"]
 
 
 class Adjoint:
-    """Reverse-mode derivative of an n-step run (mpm_hip_adjoint_*)."""
+    """Reverse-mode derivative of an n-step run (mpm_hip_adjoint_*).
+
+    `generation` counts forward runs; a reverse sweep belongs to the forward
+    run it follows (the library keeps one run's checkpoints)."""
 
     def __init__(self, cfg, particles, checkpoint_every=20):
         self.lib = load()
         self.cfg = cfg
         self.np = len(particles["x"])
+        self.generation = 0
         self.a = ctypes.c_void_p()
         _check(self.lib.mpm_hip_adjoint_create(ctypes.byref(cfg), self.np, checkpoint_every,
                                                ctypes.byref(self.a)), "mpm_hip_adjoint_create")
@@ -395,7 +402,29 @@ class Adjoint:
         self.cfg.material = mat
 
     def forward(self, nsteps):
+        self.generation += 1
         _check(self.lib.mpm_hip_adjoint_forward(self.a, nsteps), "mpm_hip_adjoint_forward")
+
+    def set_particles_device(self, ptrs):
+        """ptrs: device addresses of m, v, x, y, vx, vy, sxx, sxy, syy."""
+        _check(self.lib.mpm_hip_adjoint_set_particles_device(self.a, *ptrs),
+               "mpm_hip_adjoint_set_particles_device")
+
+    def get_device(self, name, ptr):
+        """Copies a field of the final state to device address `ptr`."""
+        _check(self.lib.mpm_hip_adjoint_get_field_device(self.a, FIELD_ID[name], ptr),
+               f"mpm_hip_adjoint_get_field_device({name})")
+
+    def backward_device(self, seed_ptrs, grad_ptrs):
+        """seed_ptrs / grad_ptrs: dict field -> device address (missing =
+        zero seed / not wanted).  Returns dL/d params as a dict."""
+        n = len(STATE_FIELDS)
+        seeds = (ctypes.c_void_p * n)(*[seed_ptrs.get(k) for k in STATE_FIELDS])
+        outs = (ctypes.c_void_p * n)(*[grad_ptrs.get(k) for k in STATE_FIELDS])
+        g = np.zeros(len(PARAMS))
+        _check(self.lib.mpm_hip_adjoint_backward_device(self.a, seeds, _dptr(g), outs),
+               "mpm_hip_adjoint_backward_device")
+        return dict(zip(PARAMS, g))
 
     def get(self, name):
         out = np.empty(self.np, dtype=np.float64)

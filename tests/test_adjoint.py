@@ -282,3 +282,189 @@ def test_adjoint_workload_gradient():
         fd = (loss(dict(w["material"], **{k: base + hs}))
               - loss(dict(w["material"], **{k: base - hs}))) / (2 * hs)
         fd_check(k, gp[k], fd)
+
+
+# ---------------------------------------------------------------------------
+# The adjoint workload as specified: loss = squared error of the final
+# runout front and particle centroid (calibrate.py), gradients with respect
+# to mu_s, E and the initial velocities, and the calibration of mu_s and E.
+
+def _c5_target(w):
+    import calibrate
+    s = pympm.Sim(gpu_config(w, workloads.ADJOINT_TARGET), w["particles"])
+    s.step(w["steps"])
+    target, _ = calibrate.observables(s.get("x"), s.get("y"), s.active())
+    s.close()
+    return target
+
+
+def _c5_loss(a, w, target, material=None, particles=None):
+    import calibrate
+    if material is not None:
+        a.set_material(**material)
+    if particles is not None:
+        a.set_particles(particles)
+    a.forward(w["steps"])
+    return calibrate.loss_and_seed(a.get("x"), a.get("y"), a.active(), target)
+
+
+@pytest.mark.gpu
+def test_c5_gradient_against_finite_differences():
+    """dL/dmu_s and dL/dE against central differences with a relative step
+    of 1e-5, and dL/d(initial vx, vy) of the front particle and of the
+    particles with the largest components against steps of 1e-6 m/s; bound
+    1e-5.  (At a relative step of 1e-6 in the constants the rounding of the
+    200-step forward run is of the order of the bound: the differences of
+    the observables are then a few ulps of positions near 0.5 m.)"""
+    import calibrate
+    w = workloads.adjoint_block()
+    say(f"C5 gradient: {len(w['particles']['x'])} particles, {w['steps']} steps")
+    target = _c5_target(w)
+    a = pympm.Adjoint(gpu_config(w), w["particles"], checkpoint_every=20)
+    loss, seed = _c5_loss(a, w, target)
+    _, front = calibrate.observables(a.get("x"), a.get("y"), a.active())
+    gp, gi = a.backward(seed)
+    base = dict(w["material"])
+    for k in ("mu_s", "E"):
+        for rel_step in (1e-5, 1e-6):
+            hs = rel_step * abs(base[k])
+            lp, _ = _c5_loss(a, w, target, material=dict(base, **{k: base[k] + hs}))
+            lm, _ = _c5_loss(a, w, target, material=dict(base, **{k: base[k] - hs}))
+            fd = (lp - lm) / (2 * hs)
+            if rel_step == 1e-5:
+                fd_check(f"{k} (h = {rel_step:g} |{k}|)", gp[k], fd)
+            else:
+                say(f"  {k} (h = {rel_step:g} |{k}|): rel {abs(gp[k] - fd) / abs(fd):.2e} "
+                    "(reported)")
+    a.set_material(**base)
+    picks = [("vx", front)] + [("vx", int(i)) for i in np.argsort(-np.abs(gi["vx"]))[:2]] \
+        + [("vy", int(i)) for i in np.argsort(-np.abs(gi["vy"]))[:2]]
+    for f, i in picks:
+        vals = []
+        for sgn in (1, -1):
+            p2 = {q: v.copy() for q, v in w["particles"].items()}
+            p2[f][i] += sgn * 1e-6
+            vals.append(_c5_loss(a, w, target, particles=p2)[0])
+        fd_check(f"{f}0[{i}]", gi[f][i], (vals[0] - vals[1]) / 2e-6)
+    a.close()
+
+
+@pytest.mark.gpu
+def test_c5_gradient_matches_cpu_adjoint():
+    """The first 20 steps of the workload: the GPU reverse sweep of the
+    front-and-centroid loss against the FP64 CPU adjoint (numpy
+    restatement over the oracle's recorded run)."""
+    import calibrate
+    w = dict(workloads.adjoint_block(), steps=20)
+    say(f"C5 vs CPU adjoint: {len(w['particles']['x'])} particles, 20 steps")
+    recs, final = AO.record(w, 20)
+    target = np.array([0.75, 0.5, 0.25])
+    _, seed = calibrate.loss_and_seed(final["x"], final["y"], final["active"], target)
+    lam0, grad = AO.adjoint(recs, seed, w["material"], w["N"], w["h"], w["dt"])
+    a = pympm.Adjoint(gpu_config(w), w["particles"], checkpoint_every=20)
+    loss, gseed = _c5_loss(a, w, target)
+    assert all(np.array_equal(gseed[k], seed[k]) for k in seed)
+    gp, gi = a.backward(gseed)
+    a.close()
+    for k in ("mu_s", "E"):
+        i = AO.PARAMS.index(k)
+        rel = abs(gp[k] - grad[i]) / abs(grad[i])
+        say(f"  d/d{k}: gpu {gp[k]:.12e} cpu {grad[i]:.12e} rel {rel:.1e}")
+        assert rel <= PARITY_TOL
+    for k in ("vx", "vy"):
+        scale = np.abs(lam0[k]).max()
+        assert np.abs(gi[k] - lam0[k]).max() <= PARITY_TOL * scale, k
+
+
+@pytest.mark.gpu
+def test_c5_calibration_recovers_mu_s_and_E():
+    """Gauss-Newton on the front and centroid from mu_s = 0.38, E = 1e6
+    recovers the target run's mu_s = 0.42 and E = 2e6."""
+    import calibrate
+    w = workloads.adjoint_block()
+    target = _c5_target(w)
+    a = pympm.Adjoint(gpu_config(w), w["particles"], checkpoint_every=20)
+    out = calibrate.calibrate(a, w["steps"], target, w["material"]["mu_s"], w["material"]["E"],
+                              log=say)
+    a.close()
+    tgt = workloads.ADJOINT_TARGET
+    assert abs(out["mu_s"] / tgt["mu_s"] - 1) <= 1e-6
+    assert abs(out["E"] / tgt["E"] - 1) <= 1e-6
+    assert out["iterations"] <= 10
+
+
+@pytest.mark.gpu
+def test_torch_bridge_rejects_a_stale_backward():
+    """A second forward through the same Adjoint before the first node's
+    backward: that backward raises instead of differentiating the second
+    run."""
+    torch = pytest.importorskip("torch")
+    import usl_autograd as ag
+    w = small_case(seed=9)
+    p = w["particles"]
+    a = pympm.Adjoint(gpu_config(w), p, checkpoint_every=5)
+    mat = w["material"]
+
+    def run(scale):
+        params = torch.tensor([mat[k] for k in pympm.PARAMS], dtype=torch.float64,
+                              requires_grad=True)
+        ins = [torch.tensor(p[k] * (scale if k == "vx" else 1.0), dtype=torch.float64,
+                            requires_grad=True)
+               for k in ("x", "y", "vx", "vy", "v", "sxx", "sxy", "syy")]
+        outs = ag.UslRun.apply(a, 6, torch.tensor(p["m"]), params, *ins)
+        return (outs[0] ** 2).sum()
+
+    first = run(1.0)
+    second = run(2.0)
+    with pytest.raises(RuntimeError, match="forward again"):
+        first.backward()
+    second.backward()      # the live run's backward works
+    a.close()
+
+
+DEVICE_BRIDGE_SCRIPT = r"""
+import sys
+import torch                     # first: the library binds to torch's HIP runtime
+assert torch.cuda.is_available()
+sys.path[:0] = ["mpm-2d_amd", "oracle", "tests"]
+import numpy as np
+import pympm
+import usl_autograd as ag
+from test_adjoint import small_case, gpu_config
+w = small_case(seed=9); p = w["particles"]; mat = w["material"]
+res = {}
+for dev in ("cpu", "cuda"):
+    a = pympm.Adjoint(gpu_config(w), p, checkpoint_every=5)
+    params = torch.tensor([mat[k] for k in pympm.PARAMS], dtype=torch.float64, device=dev,
+                          requires_grad=True)
+    ins = [torch.tensor(p[k], dtype=torch.float64, device=dev, requires_grad=True)
+           for k in ("x", "y", "vx", "vy", "v", "sxx", "sxy", "syy")]
+    outs = ag.UslRun.apply(a, 12, torch.tensor(p["m"], device=dev), params, *ins)
+    assert all(o.device.type == dev for o in outs)
+    loss = ((outs[0] - 0.3) ** 2).sum() + ((outs[1] - 0.2) ** 2).sum() + (outs[5] ** 2).sum()
+    loss.backward()
+    res[dev] = [params.grad] + [t.grad for t in ins] + [o.detach() for o in outs]
+    assert all(t.device.type == dev for t in res[dev])
+    a.close()
+same = [bool(np.array_equal(a.cpu().numpy(), b.cpu().numpy()))
+        for a, b in zip(res["cpu"], res["cuda"])]
+print("device bridge bitwise equal to the host bridge:", same, flush=True)
+assert all(same)
+"""
+
+
+@pytest.mark.gpu
+def test_torch_bridge_device_resident():
+    """The bridge with CUDA tensors (device-to-device copies) against the
+    same run with CPU tensors (host staging): gradients and outputs bitwise
+    equal.  In a fresh process that imports torch before the library, so
+    both use one HIP runtime (this test process loaded the library first)."""
+    pytest.importorskip("torch")
+    import os
+    import subprocess
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    say("case torch bridge with CUDA tensors, fresh process")
+    r = subprocess.run([sys.executable, "-c", DEVICE_BRIDGE_SCRIPT], cwd=repo, timeout=300,
+                       capture_output=True, text=True)
+    say(r.stdout[-2000:] + r.stderr[-2000:])
+    assert r.returncode == 0
