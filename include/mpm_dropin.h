@@ -10,6 +10,8 @@
 #ifndef MPM_DROPIN_H
 #define MPM_DROPIN_H
 
+#include <stdio.h>
+
 #include "mpm_abi.h"
 
 #ifdef __cplusplus
@@ -69,6 +71,42 @@ void tensor_scale3(double *A, double c);
 void tensor_sym3(double *C, const double *A);
 void tensor_skw3(double *C, const double *A);
 void tensor_contraction3(double *c, const double *A, const double *B);
+
+/* ------------------------------------------------------------------------
+ * Extensions beyond the reference's libmpm, for this repository's driver
+ * (mpm-2d_amd/host/mpm_usl_run.c): frames taken by snapshot, so the driver
+ * formats and writes frame k on a thread of its own while the steps after
+ * it run.  The reference driver writes each frame synchronously from the
+ * job's AoS (src/main.c:955-973, src/writer.c:190-201); the file written
+ * from a snapshot is byte-identical.
+ * ---------------------------------------------------------------------- */
+#define MPM_GPU_FRAME_FIELDS 12
+typedef struct mpm_gpu_frame {
+    /* page-locked host copies of the fields write_frame prints that the
+     * device advances (x_t, y_t: the velocity; state9, state10: gammap and
+     * gammadotp) */
+    double *v, *x, *y, *x_t, *y_t, *sxx, *sxy, *syy, *ux, *uy, *state9, *state10;
+    int *active;
+    size_t frame;
+    double t;
+    int state_from_job;     /* state[9], state[10] come from the job's AoS */
+} mpm_gpu_frame_t;
+
+/* on = 1: the step no longer copies particles back at frame times (only at
+ * the end of the run); the driver takes each frame with mpm_gpu_frame_begin. */
+void mpm_gpu_async_frames(job_t *job, int on);
+int mpm_gpu_frame_alloc(job_t *job, mpm_gpu_frame_t *f);
+void mpm_gpu_frame_free(mpm_gpu_frame_t *f);
+/* From the serial section at a frame time, after the step: starts the
+ * snapshot of the state after the steps so far and returns 1 at once; 0
+ * when the frame must be written from the job's AoS now (a host material
+ * plugin keeps the stress there). */
+int mpm_gpu_frame_begin(job_t *job, mpm_gpu_frame_t *f, size_t frame, double t);
+/* Waits for the snapshot (from any thread) and writes the frame in the
+ * format of write_frame; m, color and corners come from the AoS.  The
+ * particle lines are formatted in up to 16 chunks on threads of its own and
+ * written in order (the same bytes).  Returns 0, or -1 if a write failed. */
+int mpm_gpu_frame_write(FILE *fd, const job_t *job, const mpm_gpu_frame_t *f);
 
 #ifdef __cplusplus
 }

@@ -245,6 +245,26 @@ int mpm_hip_time_span(mpm_hip_ctx *ctx, int n, double *ms);
 
 int mpm_hip_destroy(mpm_hip_ctx *ctx);
 
+/* Frame output that does not stop the steps: the copy-out behind
+ * write_frame (src/writer.c:190-201; the driver writes a frame whenever
+ * t >= frame / sample_rate, src/main.c:955-973).  Copies particle fields
+ * fields[0..nfields) (mpm_hip_field) and the active flags, as they stand
+ * after the steps queued so far, into a device staging area on the
+ * context's stream, then into host[k] and active_host (NULL: not wanted) on
+ * a second stream, and returns at once: steps queued afterwards run while
+ * the transfer is in flight.  Host arrays of num_particles entries, best
+ * page-locked (mpm_hip_host_alloc) so the transfer is asynchronous.  One
+ * snapshot is in flight at a time: a second begin first waits for the
+ * previous one's transfer.  mpm_hip_snapshot_wait blocks until the last
+ * begun snapshot is in the host arrays; it may be called from a thread other
+ * than the one stepping the context, but not concurrently with a begin. */
+int mpm_hip_snapshot_begin(mpm_hip_ctx *ctx, const int *fields, int nfields,
+                           double *const *host, int *active_host);
+int mpm_hip_snapshot_wait(mpm_hip_ctx *ctx);
+/* Page-locked host memory (hipHostMalloc) for snapshot buffers. */
+int mpm_hip_host_alloc(size_t bytes, void **out);
+int mpm_hip_host_free(void *p);
+
 const char *mpm_hip_status_string(int status);
 
 /* ---------------------------------------------------------------------

@@ -32,6 +32,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "mpm_abi.h"
 #include "mpm_dropin.h"
@@ -57,6 +58,7 @@ typedef struct gpu_job {
     double *dbuf;
     int *ibuf;
     double bx, by;        /* body force the next step uses */
+    int async_frames;     /* the driver takes frames by snapshot (mpm_gpu_async_frames) */
 } gpu_job_t;
 
 static gpu_job_t G;
@@ -724,7 +726,12 @@ static void serial_step(job_t *job)
     /* the loads the next step uses, as time_varying_loads will compute them */
     G.bx = 0;
     G.by = loading_gravity(job);
-    if (G.host_material || driver_reads_after_step(job)) {
+    /* with snapshot frames only the end of the run is copied back here */
+    int reads = driver_reads_after_step(job);
+    if (G.async_frames && !G.host_material && job->t < job->t_stop) {
+        reads = 0;
+    }
+    if (G.host_material || reads) {
         CHECK(mpm_hip_sync(G.ctx));
         download_kinematics();
         if (!G.host_material) {
@@ -767,4 +774,153 @@ void explicit_mpm_step_usl_threaded(void *task)
 void explicit_mpm_step_musl_threaded(void *task)
 {
     step_threaded(task, MPM_SCHEME_MUSL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Frames by snapshot (an extension for this repository's driver)            */
+/* ------------------------------------------------------------------------ */
+void mpm_gpu_async_frames(job_t *job, int on)
+{
+    (void)job;
+    G.async_frames = on != 0;
+}
+
+int mpm_gpu_frame_alloc(job_t *job, mpm_gpu_frame_t *f)
+{
+    memset(f, 0, sizeof *f);
+    const size_t n = job->num_particles ? job->num_particles : 1;
+    double **cols[MPM_GPU_FRAME_FIELDS] = {&f->v, &f->x, &f->y, &f->x_t, &f->y_t, &f->sxx,
+                                           &f->sxy, &f->syy, &f->ux, &f->uy, &f->state9,
+                                           &f->state10};
+    for (int k = 0; k < MPM_GPU_FRAME_FIELDS; k++) {
+        if (mpm_hip_host_alloc(n * sizeof(double), (void **)cols[k]) != MPM_HIP_OK) {
+            return -1;
+        }
+    }
+    if (mpm_hip_host_alloc(n * sizeof(int), (void **)&f->active) != MPM_HIP_OK) {
+        return -1;
+    }
+    return 0;
+}
+
+void mpm_gpu_frame_free(mpm_gpu_frame_t *f)
+{
+    double *cols[MPM_GPU_FRAME_FIELDS] = {f->v, f->x, f->y, f->x_t, f->y_t, f->sxx, f->sxy,
+                                          f->syy, f->ux, f->uy, f->state9, f->state10};
+    for (int k = 0; k < MPM_GPU_FRAME_FIELDS; k++) {
+        mpm_hip_host_free(cols[k]);
+    }
+    mpm_hip_host_free(f->active);
+    memset(f, 0, sizeof *f);
+}
+
+int mpm_gpu_frame_begin(job_t *job, mpm_gpu_frame_t *f, size_t frame, double t)
+{
+    if (!G.async_frames || G.host_material || !G.ready || G.job != job) {
+        return 0;   /* the job's AoS is current: write_frame now */
+    }
+    /* the fields write_frame prints (src/writer.c:169-182) that the device
+     * advances; m, color and corners stay in the AoS */
+    int fields[MPM_GPU_FRAME_FIELDS] = {MPM_F_V, MPM_F_X, MPM_F_Y, MPM_F_VX, MPM_F_VY,
+                                        MPM_F_TXX, MPM_F_TXY, MPM_F_TYY, MPM_F_UX, MPM_F_UY,
+                                        MPM_F_GAMMAP, MPM_F_GAMMADOTP};
+    double *dst[MPM_GPU_FRAME_FIELDS] = {f->v, f->x, f->y, f->x_t, f->y_t, f->sxx, f->sxy,
+                                         f->syy, f->ux, f->uy, f->state9, f->state10};
+    /* isolin keeps no plastic state on the device (isolin.c:72-116): its
+     * state[9], state[10] are the AoS values */
+    const int nf = G.s_law == 1 ? MPM_GPU_FRAME_FIELDS - 2 : MPM_GPU_FRAME_FIELDS;
+    f->state_from_job = G.s_law == 1;
+    f->frame = frame;
+    f->t = t;
+    CHECK(mpm_hip_snapshot_begin(G.ctx, fields, nf, dst, f->active));
+    return 1;
+}
+
+/* One chunk of a frame's particle lines, formatted into memory with the
+ * same fprintf calls as write_particle (src/writer.c:169-182). */
+typedef struct frame_chunk {
+    const job_t *job;
+    const mpm_gpu_frame_t *f;
+    size_t begin, end;
+    char *text;
+    size_t len;
+} frame_chunk_t;
+
+static void write_lines(FILE *fd, const job_t *job, const mpm_gpu_frame_t *f, size_t begin,
+                        size_t end)
+{
+    for (size_t i = begin; i < end; i++) {
+        const particle_t *p = &job->particles[i];
+        const double s9 = f->state_from_job ? p->state[9] : f->state9[i];
+        const double s10 = f->state_from_job ? p->state[10] : f->state10[i];
+        fprintf(fd, "%lg %lg %lg %lg %lg %lg ", p->m, f->v[i], f->x[i], f->y[i], f->x_t[i],
+                f->y_t[i]);
+        fprintf(fd, "%lg %lg %lg %lg %lg %lg %lg %lg %lg", f->sxx[i], f->sxy[i], f->syy[i],
+                f->ux[i], f->uy[i], s9, p->color, s10, (double)f->active[i]);
+        for (int k = 0; k < 4; k++) {
+            fprintf(fd, " %lg %lg", p->corners[k][0], p->corners[k][1]);
+        }
+        fprintf(fd, "\n");
+    }
+}
+
+static void *format_chunk(void *arg)
+{
+    frame_chunk_t *c = arg;
+    FILE *fd = open_memstream(&c->text, &c->len);
+    if (!fd) {
+        c->text = NULL;
+        return NULL;
+    }
+    write_lines(fd, c->job, c->f, c->begin, c->end);
+    if (fclose(fd) != 0) {
+        free(c->text);
+        c->text = NULL;
+    }
+    return NULL;
+}
+
+int mpm_gpu_frame_write(FILE *fd, const job_t *job, const mpm_gpu_frame_t *f)
+{
+    CHECK(mpm_hip_snapshot_wait(G.ctx));
+    fprintf(fd, "%zu %lg %zu\n", f->frame, f->t, job->num_particles);
+    /* the particle lines in up to 16 chunks formatted concurrently, written
+     * in order: the same bytes as one sequential pass */
+    const size_t np = job->num_particles;
+    long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    size_t nt = ncpu > 16 ? 16 : (ncpu < 1 ? 1 : (size_t)ncpu);
+    if (np < 4096 * nt) {
+        nt = np / 4096 + 1;
+    }
+    frame_chunk_t chunk[16];
+    pthread_t th[16];
+    int started[16] = {0};
+    for (size_t k = 0; k < nt; k++) {
+        chunk[k] = (frame_chunk_t){job, f, np * k / nt, np * (k + 1) / nt, NULL, 0};
+        started[k] = k > 0 && pthread_create(&th[k], NULL, format_chunk, &chunk[k]) == 0;
+    }
+    format_chunk(&chunk[0]);
+    for (size_t k = 1; k < nt; k++) {
+        if (started[k]) {
+            pthread_join(th[k], NULL);
+        } else {
+            format_chunk(&chunk[k]);
+        }
+    }
+    int rc = 0;
+    for (size_t k = 0; k < nt; k++) {
+        if (chunk[k].text) {
+            if (fwrite(chunk[k].text, 1, chunk[k].len, fd) != chunk[k].len) {
+                rc = -1;
+            }
+            free(chunk[k].text);
+        } else {
+            /* no memory for the chunk's text: format it straight out */
+            write_lines(fd, job, f, chunk[k].begin, chunk[k].end);
+        }
+    }
+    if (fflush(fd) != 0) {
+        rc = -1;
+    }
+    return rc;
 }

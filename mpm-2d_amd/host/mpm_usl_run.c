@@ -29,6 +29,13 @@
  * --check-cfg prints the settings a -c run would use and exits.
  * --musl steps with explicit_mpm_step_musl_threaded (process_usl.c:284-462;
  * the reference's loop has that call commented out, src/main.c:948).
+ * Frames are taken by snapshot: at a frame time the device copies the
+ * printed fields into page-locked host memory while the next steps run, and
+ * a writer thread formats and writes the frame (byte-identical to the
+ * synchronous write_frame of src/writer.c:190-201, the particle lines
+ * formatted in up to 16 chunks at once; with a host material
+ * plugin the frame is written synchronously from the AoS).  --sync-frames
+ * writes every frame synchronously from the AoS, as the reference does.
  * --state writes the final particle state with full-precision hex floats,
  * one particle per line:
  *   x y x_t y_t v sxx sxy syy T[ZZ] state[9] state[10] ux uy active
@@ -55,6 +62,48 @@
 
 static FILE *frames_fd;
 static int use_musl;
+static int sync_frames;
+
+/* The frame writer thread (snapshot frames): one frame in hand at a time. */
+static pthread_mutex_t wlock = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t wcond = PTHREAD_COND_INITIALIZER;
+static int writer_on, wfull, wquit;
+static mpm_gpu_frame_t wbuf;
+static job_t *wjob;
+static pthread_t writer;
+
+static void *frame_writer(void *arg)
+{
+    (void)arg;
+    pthread_mutex_lock(&wlock);
+    for (;;) {
+        while (!wfull && !wquit) {
+            pthread_cond_wait(&wcond, &wlock);
+        }
+        if (!wfull) {
+            break;
+        }
+        pthread_mutex_unlock(&wlock);
+        if (mpm_gpu_frame_write(frames_fd, wjob, &wbuf) != 0) {
+            fprintf(stderr, "mpm_usl_run: writing frame %zu failed\n", wbuf.frame);
+        }
+        pthread_mutex_lock(&wlock);
+        wfull = 0;
+        pthread_cond_broadcast(&wcond);
+    }
+    pthread_mutex_unlock(&wlock);
+    return NULL;
+}
+
+/* Waits until the writer has finished the frame in hand. */
+static void writer_idle(void)
+{
+    pthread_mutex_lock(&wlock);
+    while (wfull) {
+        pthread_cond_wait(&wcond, &wlock);
+    }
+    pthread_mutex_unlock(&wlock);
+}
 
 static int read_grid(const char *path, int *N, double *len)
 {
@@ -146,7 +195,19 @@ static void *run_until(void *arg)
         if (rc == PTHREAD_BARRIER_SERIAL_THREAD) {
             job->stepcount++;
             if (frames_fd && job->t >= (job->frame / job->output.sample_rate_hz)) {
-                write_frame(frames_fd, job->frame, job->t, job);
+                int begun = 0;
+                if (writer_on) {
+                    writer_idle();   /* the last frame is out: its buffers are free */
+                    begun = mpm_gpu_frame_begin(job, &wbuf, job->frame, job->t);
+                }
+                if (begun) {
+                    pthread_mutex_lock(&wlock);
+                    wfull = 1;
+                    pthread_cond_broadcast(&wcond);
+                    pthread_mutex_unlock(&wlock);
+                } else {
+                    write_frame(frames_fd, job->frame, job->t, job);
+                }
                 job->frame++;
             }
             time_varying_loads(job);
@@ -395,6 +456,7 @@ int main(int argc, char **argv)
         else if (!strcmp(k, "--frames")) o.frames = v, a++;
         else if (!strcmp(k, "--state")) o.state = v, a++;
         else if (!strcmp(k, "--musl")) use_musl = 1;
+        else if (!strcmp(k, "--sync-frames")) sync_frames = 1;
         else if (!strcmp(k, "--check-cfg")) o.check_cfg = 1;
         else if (k[0] != '-' && !o.have_t_stop) {
             o.t_stop = strtod(k, NULL);
@@ -601,6 +663,18 @@ int main(int argc, char **argv)
     }
     job->frame = (size_t)floor(job->t * job->output.sample_rate_hz);
 
+    if (frames_fd && !sync_frames) {
+        if (mpm_gpu_frame_alloc(job, &wbuf) == 0) {
+            mpm_gpu_async_frames(job, 1);
+            wjob = job;
+            writer_on = pthread_create(&writer, NULL, frame_writer, NULL) == 0;
+            if (!writer_on) {
+                mpm_gpu_async_frames(job, 0);
+            }
+        } else {
+            mpm_gpu_frame_free(&wbuf);
+        }
+    }
     pthread_t *th = malloc(sizeof(pthread_t) * T);
     for (size_t i = 0; i + 1 < T; i++) {
         pthread_create(&th[i], NULL, run_until, &tasks[i]);
@@ -608,6 +682,15 @@ int main(int argc, char **argv)
     run_until(&tasks[T - 1]);
     for (size_t i = 0; i + 1 < T; i++) {
         pthread_join(th[i], NULL);
+    }
+    if (writer_on) {
+        pthread_mutex_lock(&wlock);
+        wquit = 1;
+        pthread_cond_broadcast(&wcond);
+        pthread_mutex_unlock(&wlock);
+        pthread_join(writer, NULL);
+        mpm_gpu_frame_free(&wbuf);
+        mpm_gpu_async_frames(job, 0);
     }
     if (state_fd) {
         write_state(state_fd, job);

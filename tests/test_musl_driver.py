@@ -324,3 +324,33 @@ def test_cfg_restart_continues_the_run(tmp_path):
         assert err <= 1e-4 * scale, f"{k}: {err} vs scale {scale}"
     assert np.array_equal(s2["active"], ref["active"])
     assert s1["gammap"].max() > 0 and np.all(s2["gammap"] >= s1["gammap"] - 1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("plugin_dir,mat", [("lib", "g_local_mu2"), ("lib", "isolin"),
+                                            ("lib/host_only", "g_local_mu2")])
+def test_snapshot_frames_equal_synchronous_frames(tmp_path, plugin_dir, mat):
+    """Frames taken by snapshot (device copy, transfer and formatting on a
+    writer thread while the next steps run) against the reference's
+    synchronous write from the AoS (--sync-frames): the frame files are
+    byte-identical, with a frame after every step (the writer hands back
+    each frame before the next snapshot), for a device material, for isolin
+    (whose state[9], state[10] stay in the AoS) and for a host material
+    plugin (frames then written synchronously)."""
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    steps = 25
+    texts = []
+    for mode in ("snapshot", "sync"):
+        d = tmp_path / mode
+        d.mkdir()
+        kw = {"keys": ("E", "nu")} if mat == "isolin" else {}
+        path, outdir = write_cfg(d, w, rate=1e7, plugin_dir=plugin_dir, mat=mat, **kw)
+        args = [RUN, "-c", str(path), "-t", "2", repr((steps - 0.5) * w["dt"])]
+        if mode == "sync":
+            args.insert(1, "--sync-frames")
+        print(f"case frames {mode} {plugin_dir}/{mat}", flush=True)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        texts.append((outdir / "frame_particle_data.txt").read_text())
+    assert texts[0].count("\n") == steps * (1 + len(w["particles"]["x"]))
+    assert texts[0] == texts[1]
