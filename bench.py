@@ -67,14 +67,12 @@ def read_traffic(kernel, summary="hbm_traffic_silo_512.txt"):
     return None
 
 
-def cpu_baseline(w, budget_s=15.0):
-    """The CPU restatement on all host cores, until `budget_s` of stepping."""
+def _time_oracle(w, variant, threads, budget_s, log_path=None):
+    """(particle-steps/s, steps, seconds) of the CPU restatement stepping
+    for `budget_s` after one warm-up step."""
     import pyoracle
-    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
-    variant = "v4" if " avx512f" in flags else "v3"
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    model = re.search(r"model name\s*:\s*(.*)", flags)
-    o = pyoracle.from_workload(w, variant=variant, num_threads=threads)
+    o = pyoracle.from_workload(w, variant=variant, num_threads=threads,
+                               log_crossings=log_path is not None, log_path=log_path)
     o.run(1)   # first-touch of the AoS arrays and colour lists
     steps, t0 = 0, time.perf_counter()
     while True:
@@ -83,17 +81,43 @@ def cpu_baseline(w, budget_s=15.0):
         el = time.perf_counter() - t0
         if el >= budget_s or steps >= 2000:
             break
-    solver = (", nonlocal solve: banded Cholesky on the host (the reference uses "
-              "CXSparse LU, absent here)" if w.get("nonlocal_material") else "")
     nact = int(o.particles()["active"].sum())
     o.close()
-    return {"value": nact * steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"{w['name']}: {len(w['particles']['x'])} particles, {steps} steps "
-                       f"after 1 warm-up step ({el:.1f} s), oracle/usl_oracle.c x86-64-"
-                       f"{variant} build with the reference's -O3 and FP contraction, "
-                       f"{threads} pthreads{solver}"),
-            "cpu": model.group(1).strip() if model else "unknown"}
+    return nact * steps / el, steps, el
+
+
+def cpu_baseline(w, budget_s=15.0, variants=True):
+    """The CPU restatement on all host cores, until `budget_s` of stepping
+    (the reported baseline: the fastest CPU setting, logging off).  With
+    `variants`, also one thread, and all threads with the reference's
+    per-crossing log lines written to a file (libmpm/process_usl.c:651-663),
+    each for a third of the budget."""
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    variant = "v4" if " avx512f" in flags else "v3"
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = re.search(r"model name\s*:\s*(.*)", flags)
+    value, steps, el = _time_oracle(w, variant, threads, budget_s)
+    solver = (", nonlocal solve: banded Cholesky on the host (the reference uses "
+              "CXSparse LU, absent here)" if w.get("nonlocal_material") else "")
+    out = {"value": value, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": (f"{w['name']}: {len(w['particles']['x'])} particles, {steps} steps "
+                      f"after 1 warm-up step ({el:.1f} s), oracle/usl_oracle.c x86-64-"
+                      f"{variant} build with the reference's -O3 and FP contraction, "
+                      f"{threads} pthreads, crossing log off{solver}"),
+           "cpu": model.group(1).strip() if model else "unknown"}
+    if variants:
+        import tempfile
+        v1, s1, e1 = _time_oracle(w, variant, 1, budget_s / 3)
+        with tempfile.NamedTemporaryFile(prefix="mpm_crossings_", suffix=".log") as lf:
+            vl, sl, el_ = _time_oracle(w, variant, threads, budget_s / 3, log_path=lf.name)
+            log_bytes = os.path.getsize(lf.name)
+        out["variants"] = [
+            {"value": v1, "cores": 1, "logging": False,
+             "sample": f"{s1} steps ({e1:.1f} s) on 1 thread"},
+            {"value": vl, "cores": threads, "logging": True,
+             "sample": f"{sl} steps ({el_:.1f} s) on {threads} threads, {log_bytes} bytes of "
+                       "crossing log lines"}]
+    return out
 
 
 def bench_heap(args):

@@ -103,7 +103,7 @@ NONLOCAL_KEYS = ("E", "nu", "mu_s", "A", "B", "d", "rho", "p_cap", "c")
 
 def make_params(lib, N, h, dt, material=None, bc_kind=0, bc_fp64=(), bc_int=(),
                 g_mag=9.8, ramp_time=0.1, num_threads=1, log_crossings=False,
-                nonlocal_material=None, material_kind=0, scheme=0):
+                nonlocal_material=None, material_kind=0, scheme=0, log_path=None):
     """material: g_local_mu2 constants (E and nu also serve isolin);
     nonlocal_material: g_nonlocal constants (keys of NONLOCAL_KEYS) -- selects
     the g_nonlocal law when given; material_kind 2 selects isolin."""
@@ -125,6 +125,8 @@ def make_params(lib, N, h, dt, material=None, bc_kind=0, bc_fp64=(), bc_int=(),
     for i, val in enumerate(bc_int):
         q.bc_int[i] = val
     q.log_crossings = 1 if log_crossings else 0
+    if log_path is not None:
+        q.log_path = log_path.encode()
     return q
 
 
@@ -194,7 +196,8 @@ class Oracle:
             pass
 
 
-def from_workload(w, variant="parity", num_threads=1, log_crossings=False, particles=None):
+def from_workload(w, variant="parity", num_threads=1, log_crossings=False, particles=None,
+                  log_path=None):
     lib = load(variant)
     q = make_params(lib, w["N"], w["h"], w["dt"], material=w["material"],
                     bc_kind=w.get("bc_kind", 0), bc_fp64=w.get("bc_fp64", ()),
@@ -202,5 +205,6 @@ def from_workload(w, variant="parity", num_threads=1, log_crossings=False, parti
                     ramp_time=w.get("ramp_time", 0.1), num_threads=num_threads,
                     log_crossings=log_crossings,
                     nonlocal_material=w.get("nonlocal_material"),
-                    material_kind=w.get("material_kind", 0), scheme=w.get("scheme", 0))
+                    material_kind=w.get("material_kind", 0), scheme=w.get("scheme", 0),
+                    log_path=log_path)
     return Oracle(q, particles if particles is not None else w["particles"], variant=variant)
