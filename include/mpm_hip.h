@@ -309,6 +309,15 @@ int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms);
  * trajectory is unchanged by the choice.  *chosen = the capacity kept,
  * ms[3] = trial time per step for 16, 32, 64 (negative: infeasible). */
 int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, double *ms);
+/* The same search over the launch shape: bin capacity 16, 32, 64 times two
+ * or three threads per node in the P2G kernel (two: one thread folds the
+ * m, mx, fx sums of a node and one my, fy; three: m, mx, my / fx / fy, more
+ * waves per 16 x 8-node tile).  Keeps the fastest pair; *capacity,
+ * *threads_per_node = the pair kept; ms[6] = trial time per step for
+ * (16, 2), (16, 3), (32, 2), (32, 3), (64, 2), (64, 3), negative where the
+ * capacity overflows.  The trajectory does not depend on the choice. */
+int mpm_hip_batch_autotune_launch(mpm_hip_batch *b, int trial_steps, int *capacity,
+                                  int *threads_per_node, double *ms);
 int mpm_hip_batch_destroy(mpm_hip_batch *b);
 
 /* ---------------------------------------------------------------------

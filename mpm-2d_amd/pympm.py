@@ -112,6 +112,8 @@ def load():
     lib.mpm_hip_batch_sync.argtypes = [P]
     lib.mpm_hip_batch_time_steps.argtypes = [P, I, D]
     lib.mpm_hip_batch_autotune.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
+    lib.mpm_hip_batch_autotune_launch.argtypes = [P, I, ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_batch_destroy.argtypes = [P]
     DP = ctypes.POINTER(D)
     lib.mpm_hip_adjoint_create.argtypes = [ctypes.POINTER(Config), Z, I, ctypes.POINTER(P)]
@@ -343,6 +345,16 @@ class Batch:
         _check(self.lib.mpm_hip_batch_autotune(self.b, trial_steps, ctypes.byref(cap),
                                                _dptr(ms)), "mpm_hip_batch_autotune")
         return cap.value, ms
+
+    def autotune_launch(self, trial_steps=4):
+        """(capacity, threads per node, per-step trial ms [6] over capacity
+        16/32/64 x 2/3 threads per node)."""
+        cap, roles = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(6, dtype=np.float64)
+        _check(self.lib.mpm_hip_batch_autotune_launch(self.b, trial_steps, ctypes.byref(cap),
+                                                      ctypes.byref(roles), _dptr(ms)),
+               "mpm_hip_batch_autotune_launch")
+        return cap.value, roles.value, ms
 
     def close(self):
         if self.b:

@@ -166,3 +166,27 @@ def test_batch_rejects_unsupported_members():
     cfgs[0].material_kind = 1
     with pytest.raises(pympm.MpmHipError):
         pympm.Batch(cfgs, [x["particles"] for x in w])
+
+
+@pytest.mark.gpu
+def test_autotune_launch_shape_keeps_the_trajectory():
+    """The search over bin capacity x threads per node restores the state
+    after every trial and leaves the trajectory bitwise unchanged, whichever
+    pair it keeps."""
+    ens = workloads.ensemble(members=12, seed=9, p_max=16384)
+    say(f"launch-shape autotune on a batch of {len(ens)}")
+    a = pympm.Batch([member_config(w) for w in ens], [w["particles"] for w in ens])
+    ref = pympm.Batch([member_config(w) for w in ens], [w["particles"] for w in ens])
+    a.step(9)
+    cap, roles, ms = a.autotune_launch(trial_steps=5)
+    say(f"  kept capacity {cap}, {roles} threads per node; ms/step {ms.tolist()}")
+    assert cap in (16, 32, 64) and roles in (2, 3)
+    assert ms[[16, 32, 64].index(cap) * 2 + (roles - 2)] == ms[ms > 0].min()
+    a.step(31)
+    ref.step(40)
+    a.sync()
+    ref.sync()
+    for k in range(len(ens)):
+        assert_member_equal(a, k, ref.state(k, PFIELDS), ref.active(k), "launch autotuned")
+    a.close()
+    ref.close()
