@@ -142,6 +142,9 @@ typedef struct mpm_hip_config {
                                workgroups, 0 = as many as fit (with fewer
                                workgroups than 32x32-node tiles the solve
                                takes its Jacobi-preconditioned path) */
+    int p2g_tile;           /* nodes per P2G tile: 0 = chosen from the grid
+                               size, 1 = 16 x 8, 2 = 8 x 4 (no effect on
+                               results; ensembles use 16 x 8) */
 } mpm_hip_config;
 
 enum mpm_hip_scheme {

@@ -63,7 +63,8 @@ class Config(ctypes.Structure):
                 ("flags", ctypes.c_int), ("device", ctypes.c_int),
                 ("material_kind", ctypes.c_int), ("nl", Nonlocal),
                 ("cg_rel_tol", ctypes.c_double), ("cg_max_iter", ctypes.c_int),
-                ("scheme", ctypes.c_int), ("cg_max_blocks", ctypes.c_int)]
+                ("scheme", ctypes.c_int), ("cg_max_blocks", ctypes.c_int),
+                ("p2g_tile", ctypes.c_int)]
 
 
 _lib = None
@@ -143,11 +144,12 @@ def _check(code, what):
 def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
                 g_mag=9.8, ramp_time=0.1, t0=0.0, bin_capacity=0, flags=0, device=0,
                 nonlocal_material=None, cg_rel_tol=1e-10, cg_max_iter=20000, material_kind=0,
-                scheme=0, cg_max_blocks=0):
+                scheme=0, cg_max_blocks=0, p2g_tile=0):
     """nonlocal_material: g_nonlocal constants (E, nu, mu_s, A, B, d, rho,
     p_cap, c); selects the g_nonlocal law when given.  material_kind:
     MAT_G_LOCAL_MU2, MAT_G_NONLOCAL, MAT_ISOLIN or MAT_DP_RI_MT.  cg_max_blocks
-    caps the solve's persistent workgroups (0 = as many as fit)."""
+    caps the solve's persistent workgroups (0 = as many as fit).  p2g_tile:
+    nodes per P2G tile (0 chosen from the grid, 1 16 x 8, 2 8 x 4)."""
     lib = load()
     cfg = Config()
     lib.mpm_hip_default_config(ctypes.byref(cfg))
@@ -159,6 +161,7 @@ def make_config(N, h, dt, material=None, bc_kind=BC_NONE, bc_fp64=(), bc_int=(),
             setattr(cfg.nl, k, val)
     cfg.cg_rel_tol, cfg.cg_max_iter = cg_rel_tol, cg_max_iter
     cfg.cg_max_blocks = cg_max_blocks
+    cfg.p2g_tile = p2g_tile
     cfg.N, cfg.h, cfg.dt, cfg.t0, cfg.loads_t0 = N, h, dt, t0, t0
     cfg.g_mag, cfg.ramp_time = g_mag, ramp_time
     for k, val in (material or {}).items():

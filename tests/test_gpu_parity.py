@@ -201,6 +201,52 @@ def test_bin_overflow_is_reported():
     sim.sync()
 
 
+SHAPE_CASES = {
+    # 4 particles per cell: one staging batch per 8 x 4 tile
+    "silo_64": lambda: dict(workloads.silo(seed=5, cells=64), steps=60),
+    # 36 per cell: a full 9 x 5-cell region (1,620 records) spans batches
+    "dense_36": lambda: dict(workloads.small_box(cells=32, bx=10, by=12, steps=30), cap=64,
+                             particles=workloads.block(10, 12, 32, seed=13, ppc_side=6,
+                                                       jitter=0.02, cell_x0=11, cell_y0=2)),
+    # 64 per cell: one colour of a region (15 cells) spans several runs
+    "dense_64": lambda: dict(workloads.small_box(cells=32, bx=8, by=8, steps=8), cap=64,
+                             particles=workloads.block(8, 8, 32, seed=17, ppc_side=8,
+                                                       jitter=0.02, cell_x0=12, cell_y0=3)),
+}
+
+
+@pytest.mark.parametrize("scheme", [pympm.SCHEME_USL, pympm.SCHEME_MUSL])
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("case", sorted(SHAPE_CASES))
+def test_p2g_tile_shapes_bitwise(case, tile, scheme):
+    """Both P2G tile shapes (16 x 8 nodes, colours staged one by one; 8 x 4
+    nodes, colours merged into shared staging batches), forced through the
+    config on the same inputs, in the USL and MUSL steps: bitwise against
+    the oracle."""
+    w = dict(SHAPE_CASES[case](), scheme=scheme)
+    p = w["particles"]
+    say(f"case {case}: tile {tile} scheme {scheme} np={len(p['x'])} steps={w['steps']}")
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
+                            g_mag=w["g_mag"], ramp_time=w["ramp_time"], scheme=scheme,
+                            bin_capacity=w.get("cap", 0), p2g_tile=tile)
+    sim = pympm.Sim(cfg, p)
+    sim.step(w["steps"])
+    sim.sync()
+    o = pyoracle.from_workload(w, num_threads=4, particles=p)
+    o.run(w["steps"])
+    compare({k: sim.get(k) for k in PFIELDS}, o.particles(), PFIELDS,
+            f"{case} tile {tile} scheme {scheme}", tol=REL_TOL_100)
+    compare({k: sim.node(k) for k in NFIELDS}, o.nodes(), NFIELDS, f"{case} nodes")
+
+
+def test_p2g_tile_config_is_validated():
+    w = workloads.small_box(cells=16, bx=5, by=9)
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"], p2g_tile=3)
+    with pytest.raises(pympm.MpmHipError):
+        pympm.Sim(cfg, w["particles"])
+
+
 def test_column_collapse_1000_steps():
     """The 64 x 64 column collapse (8,192 particles), all 1000 steps."""
     w = workloads.column_collapse()
