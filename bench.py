@@ -192,9 +192,10 @@ def bench_ensemble(args, emit=True):
                               g_mag=w["g_mag"], ramp_time=w["ramp_time"]) for w in ens]
     b = pympm.Batch(cfgs, [w["particles"] for w in ens])
     b.step(args.warmup)
-    cap, roles, trial = b.autotune_launch(trial_steps=4)
-    log(f"bench: autotuned bin capacity {cap}, {roles} threads per node (ms/step for "
-        f"16/32/64 x 2/3: {trial.tolist()})")
+    tile, cap, roles, trial = b.autotune_tiles(trial_steps=4)
+    tile_name = {1: "16x8", 2: "8x4"}[tile]
+    log(f"bench: autotuned P2G tiles {tile_name} nodes, bin capacity {cap}, {roles} threads per "
+        f"node (ms/step for 16x8 then 8x4, each 16/32/64 x 2/3: {trial.tolist()})")
     nact0 = sum(int(b.active(k).sum()) for k in range(len(ens)))
     t0 = time.perf_counter()
     ms = b.time_steps(args.steps)
@@ -249,7 +250,8 @@ def bench_ensemble(args, emit=True):
                                f"{np_sum} particles, mu_s~U[0.3,0.5], E~logU[5e5,5e6], "
                                "dt=1e-5",
                    "members": len(ens), "particles": np_sum, "nodes": nn_sum,
-                   "bin_capacity": cap, "threads_per_node": roles,
+                   "p2g_tile_nodes": tile_name, "bin_capacity": cap,
+                   "threads_per_node": roles,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -259,7 +261,8 @@ def bench_ensemble(args, emit=True):
         "step_roofline": {"algorithmic_bytes": step_bytes,
                           "achieved_GBs": step_bytes / (step_ms * 1e-3) / 1e9,
                           "frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "autotune_ms_per_step": {f"cap{c}_threads{r}": float(trial[2 * k + j])
+        "autotune_ms_per_step": {f"tile{t}_cap{c}_threads{r}": float(trial[6 * i + 2 * k + j])
+                                 for i, t in enumerate(("16x8", "8x4"))
                                  for k, c in enumerate((16, 32, 64))
                                  for j, r in enumerate((2, 3))},
         "plain": {"value": plain_value, "steps": plain_steps,

@@ -144,7 +144,9 @@ typedef struct mpm_hip_config {
                                takes its Jacobi-preconditioned path) */
     int p2g_tile;           /* nodes per P2G tile: 0 = chosen from the grid
                                size, 1 = 16 x 8, 2 = 8 x 4 (no effect on
-                               results; ensembles use 16 x 8) */
+                               results; an ensemble takes cfgs[0]'s, 0
+                               meaning 16 x 8, and its autotuner can
+                               switch) */
 } mpm_hip_config;
 
 enum mpm_hip_scheme {
@@ -315,12 +317,23 @@ int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, doubl
 /* The same search over the launch shape: bin capacity 16, 32, 64 times two
  * or three threads per node in the P2G kernel (two: one thread folds the
  * m, mx, fx sums of a node and one my, fy; three: m, mx, my / fx / fy, more
- * waves per 16 x 8-node tile).  Keeps the fastest pair; *capacity,
- * *threads_per_node = the pair kept; ms[6] = trial time per step for
- * (16, 2), (16, 3), (32, 2), (32, 3), (64, 2), (64, 3), negative where the
- * capacity overflows.  The trajectory does not depend on the choice. */
+ * waves per tile), with the batch's current P2G tiles.  Keeps the fastest
+ * pair; *capacity, *threads_per_node = the pair kept; ms[6] = trial time per
+ * step for (16, 2), (16, 3), (32, 2), (32, 3), (64, 2), (64, 3), negative
+ * where the capacity overflows.  The trajectory does not depend on the
+ * choice. */
 int mpm_hip_batch_autotune_launch(mpm_hip_batch *b, int trial_steps, int *capacity,
                                   int *threads_per_node, double *ms);
+/* The search over P2G tile shape as well: 16 x 8-node tiles (the colours
+ * staged one after another) and 8 x 4-node tiles (all four colours of a
+ * tile staged at once), each with the six pairs above.  *tile = the shape
+ * kept, as mpm_hip_config.p2g_tile codes it (1: 16 x 8, 2: 8 x 4);
+ * ms[12] = trial time per step, the six pairs of 16 x 8 then of 8 x 4.  The
+ * batch's initial shape is cfgs[0].p2g_tile (2: 8 x 4, otherwise 16 x 8).
+ * The trajectory does not depend on the choice.  (The reference runs one
+ * job per process and has no ensemble; this has no counterpart there.) */
+int mpm_hip_batch_autotune_tiles(mpm_hip_batch *b, int trial_steps, int *tile, int *capacity,
+                                 int *threads_per_node, double *ms);
 int mpm_hip_batch_destroy(mpm_hip_batch *b);
 
 /* ---------------------------------------------------------------------

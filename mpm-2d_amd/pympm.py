@@ -115,6 +115,9 @@ def load():
     lib.mpm_hip_batch_autotune.argtypes = [P, I, ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_batch_autotune_launch.argtypes = [P, I, ctypes.POINTER(ctypes.c_int),
                                                   ctypes.POINTER(ctypes.c_int), D]
+    lib.mpm_hip_batch_autotune_tiles.argtypes = [P, I, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int), D]
     lib.mpm_hip_batch_destroy.argtypes = [P]
     DP = ctypes.POINTER(D)
     lib.mpm_hip_adjoint_create.argtypes = [ctypes.POINTER(Config), Z, I, ctypes.POINTER(P)]
@@ -358,6 +361,18 @@ class Batch:
                                                       ctypes.byref(roles), _dptr(ms)),
                "mpm_hip_batch_autotune_launch")
         return cap.value, roles.value, ms
+
+    def autotune_tiles(self, trial_steps=4):
+        """(tile shape as p2g_tile codes it: 1 16 x 8, 2 8 x 4; capacity;
+        threads per node; per-step trial ms [12]: 16 x 8 then 8 x 4, each
+        over capacity 16/32/64 x 2/3 threads per node)."""
+        tile, cap, roles = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(12, dtype=np.float64)
+        _check(self.lib.mpm_hip_batch_autotune_tiles(self.b, trial_steps, ctypes.byref(tile),
+                                                     ctypes.byref(cap), ctypes.byref(roles),
+                                                     _dptr(ms)),
+               "mpm_hip_batch_autotune_tiles")
+        return tile.value, cap.value, roles.value, ms
 
     def close(self):
         if self.b:

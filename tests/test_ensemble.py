@@ -24,10 +24,11 @@ def say(msg):
     sys.stdout.flush()
 
 
-def member_config(w, cap=0):
+def member_config(w, cap=0, tile=0):
     return pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
                              bc_kind=w["bc_kind"], bc_fp64=w["bc_fp64"], bc_int=w["bc_int"],
-                             g_mag=w["g_mag"], ramp_time=w["ramp_time"], bin_capacity=cap)
+                             g_mag=w["g_mag"], ramp_time=w["ramp_time"], bin_capacity=cap,
+                             p2g_tile=tile)
 
 
 def same(a, b):
@@ -106,11 +107,13 @@ def test_batch_matches_single_contexts():
 
 
 @pytest.mark.gpu
-def test_batch_particles_crossing_into_empty_tiles():
+@pytest.mark.parametrize("tile", [1, 2])
+def test_batch_particles_crossing_into_empty_tiles(tile):
     """Members whose blocks fly across tile rows and columns: tiles whose
     regions were empty at upload (checked sixteen at a time by the light-tile
     kernel) fill up and empty again, and tiles occupied at upload empty out;
-    every member matches the oracle bitwise."""
+    every member matches the oracle bitwise, with 16 x 8- and 8 x 4-node
+    P2G tiles."""
     members = []
     for k, (n, cx0, cy0, vx, vy) in enumerate([(48, 4, 20, 0.4, -0.1), (64, 30, 30, -0.35, 0.2),
                                                (40, 2, 2, 0.3, 0.3)]):
@@ -120,8 +123,9 @@ def test_batch_particles_crossing_into_empty_tiles():
         p["vy"][:] = vy * w["h"] / w["dt"]
         members.append(dict(w, particles=p))
     steps = 50
-    say(f"batch of {len(members)} crossing tiles, {steps} steps")
-    b = pympm.Batch([member_config(w) for w in members], [w["particles"] for w in members])
+    say(f"batch of {len(members)} crossing tiles, {steps} steps, p2g_tile {tile}")
+    b = pympm.Batch([member_config(w, tile=tile) for w in members],
+                    [w["particles"] for w in members])
     b.step(steps)
     b.sync()
     for k, w in enumerate(members):
@@ -188,5 +192,50 @@ def test_autotune_launch_shape_keeps_the_trajectory():
     ref.sync()
     for k in range(len(ens)):
         assert_member_equal(a, k, ref.state(k, PFIELDS), ref.active(k), "launch autotuned")
+    a.close()
+    ref.close()
+
+
+@pytest.mark.gpu
+def test_batch_small_tiles_match_single_contexts():
+    """The batch on 8 x 4-node P2G tiles (all four colours of a tile staged
+    at once) against single-job contexts, over graph replays from both bin
+    parities."""
+    ens = workloads.ensemble(members=16, seed=6, p_max=16384)
+    steps = 40
+    say(f"batch of {len(ens)} on 8 x 4 tiles vs single contexts, {steps} steps")
+    b = pympm.Batch([member_config(w, tile=2) for w in ens], [w["particles"] for w in ens])
+    for chunk in (3, 8, 29):
+        b.step(chunk)
+    b.sync()
+    for k, w in enumerate(ens):
+        s = pympm.Sim(member_config(w), w["particles"])
+        s.step(steps)
+        s.sync()
+        assert_member_equal(b, k, s.state(PFIELDS), s.active(), "8 x 4 batch")
+        s.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_autotune_tiles_keeps_the_trajectory():
+    """The search over P2G tile shape x bin capacity x threads per node
+    (twelve trials, each from the restored state; a shape switch resets the
+    tiles' zero flags) leaves the trajectory bitwise unchanged."""
+    ens = workloads.ensemble(members=12, seed=10, p_max=16384)
+    say(f"tile-shape autotune on a batch of {len(ens)}")
+    a = pympm.Batch([member_config(w) for w in ens], [w["particles"] for w in ens])
+    ref = pympm.Batch([member_config(w) for w in ens], [w["particles"] for w in ens])
+    a.step(11)
+    tile, cap, roles, ms = a.autotune_tiles(trial_steps=5)
+    say(f"  kept tile {tile}, capacity {cap}, {roles} threads per node; ms/step {ms.tolist()}")
+    assert tile in (1, 2) and cap in (16, 32, 64) and roles in (2, 3)
+    assert ms[(tile - 1) * 6 + [16, 32, 64].index(cap) * 2 + (roles - 2)] == ms[ms > 0].min()
+    a.step(29)
+    ref.step(40)
+    a.sync()
+    ref.sync()
+    for k in range(len(ens)):
+        assert_member_equal(a, k, ref.state(k, PFIELDS), ref.active(k), "tile autotuned")
     a.close()
     ref.close()
