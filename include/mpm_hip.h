@@ -238,9 +238,12 @@ int mpm_hip_get_crossings(mpm_hip_ctx *ctx, long long *count);
 /* g_nonlocal: iterations and final relative residual of the last solve. */
 int mpm_hip_get_solver_stats(mpm_hip_ctx *ctx, int *iterations, double *rel_residual);
 
-/* Runs n steps and times them with HIP events recorded on the context's
- * stream.  ms[0] = whole span, ms[1] = sum of P2G+grid kernel times,
- * ms[2] = sum of G2P+stress kernel times (per-kernel events). */
+/* Runs n steps and times them with HIP events on the context's stream.
+ * ms[0] = whole span, ms[1] = sum of the P2G+grid kernel times, ms[2] = sum
+ * of the G2P+stress kernel times (for MUSL and g_nonlocal: of the rest of
+ * each step).  Kernel times come from events the kernel's own dispatch
+ * records (hipExtLaunchKernelGGL), the rest of a MUSL or g_nonlocal step is
+ * bracketed by recorded events. */
 int mpm_hip_time_steps(mpm_hip_ctx *ctx, int n, double *ms);
 /* Runs n steps between two HIP events on the context's stream (created
  * before the first is recorded) and no others: ms[0] = the span, the
@@ -304,9 +307,9 @@ int mpm_hip_batch_get_time(mpm_hip_batch *b, int member, double *t, double *by);
 int mpm_hip_batch_step(mpm_hip_batch *b, int n);
 int mpm_hip_batch_sync(mpm_hip_batch *b);
 /* Runs n steps timed with HIP events on the batch's stream: ms[0] = span
- * of all n steps; ms[1], ms[2] = P2G+grid and G2P+stress kernel durations
- * of the last step, which is launched outside the graph with an event
- * between its two kernels. */
+ * of all n steps; ms[1], ms[2] = P2G+grid (its three kernels) and
+ * G2P+stress durations of the last step, which is launched outside the
+ * graph (G2P's start and end events recorded by its dispatch). */
 int mpm_hip_batch_time_steps(mpm_hip_batch *b, int n, double *ms);
 /* Picks the fastest bin capacity among 16, 32 and 64 that holds every
  * cell: runs `trial_steps` steps per candidate from the current state,

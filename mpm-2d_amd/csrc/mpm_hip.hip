@@ -44,6 +44,7 @@
 //                             scratch array (no allocation per call)
 
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
