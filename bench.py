@@ -69,6 +69,14 @@ def read_traffic(kernel, summary="hbm_traffic_silo_512.txt"):
     return None
 
 
+def _traffic_rate(traffic_bytes, kernel_ms):
+    """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    calibrated) over the launch's duration, GB/s; None without a summary."""
+    if traffic_bytes is None or not kernel_ms:
+        return None
+    return traffic_bytes / (kernel_ms * 1e-3) / 1e9
+
+
 def _time_oracle(w, variant, threads, budget_s, log_path=None):
     """(particle-steps/s, steps, seconds) of the CPU restatement stepping
     for `budget_s` after one warm-up step."""
@@ -258,6 +266,8 @@ def bench_ensemble(args, emit=True):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(dom, "hbm_traffic_ensemble_256.txt"),
+                     "traffic_GBs": _traffic_rate(read_traffic(dom, "hbm_traffic_ensemble_256.txt"),
+                                                  dom_ms),
                      "kernel": dom, "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
         "kernels_ms": {k: v[0] for k, v in kernels.items()},
         "step_roofline": {"algorithmic_bytes": step_bytes,
@@ -616,7 +626,8 @@ def bench_silo(args):
                    "particles": npart, "nodes": nnodes, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic(dom), "kernel": dom,
+                     "traffic": read_traffic(dom),
+                     "traffic_GBs": _traffic_rate(read_traffic(dom), dom_ms), "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
         "kernels_ms": {k: v[0] for k, v in kernels.items()},
         "step_roofline": {"algorithmic_bytes": step_bytes,
