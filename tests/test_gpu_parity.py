@@ -297,3 +297,35 @@ def test_full_size_silo_properties_after_200_steps():
     assert act.sum() == len(act)
     assert nm.sum() == pytest.approx(w["particles"]["m"].sum(), rel=1e-12)
     assert np.all(runs[0]["vy"] <= 1e-9)
+
+
+@pytest.mark.parametrize("case", ["usl", "musl", "nonlocal"])
+def test_timed_steps_follow_the_same_trajectory(case):
+    """Steps taken through the timing calls (the per-kernel pass, whose P2G
+    and plain G2P dispatches record their own events, and the span) follow
+    the trajectory of plain steps bitwise, and report consistent times:
+    each part's time positive and their sum within the span."""
+    if case == "nonlocal":
+        w = workloads.heap(cells=32)
+        extra = dict(nonlocal_material=w["nonlocal_material"], cg_rel_tol=1e-12,
+                     cg_max_iter=50000)
+    else:
+        w = workloads.silo(seed=5, cells=64)
+        extra = dict(scheme=pympm.SCHEME_MUSL if case == "musl" else pympm.SCHEME_USL)
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"],
+                            bc_kind=w.get("bc_kind", 0), bc_fp64=w.get("bc_fp64", ()),
+                            bc_int=w.get("bc_int", ()), g_mag=w["g_mag"],
+                            ramp_time=w["ramp_time"], **extra)
+    say(f"case timed steps, {case}")
+    a = pympm.Sim(cfg, w["particles"])
+    b = pympm.Sim(cfg, w["particles"])
+    ms = a.time_steps(12)
+    span = a.time_span(8)
+    b.step(20)
+    b.sync()
+    compare({k: a.get(k) for k in PFIELDS}, {k: b.get(k) for k in PFIELDS}, PFIELDS,
+            "timed vs plain steps")
+    assert ms[1] > 0 and ms[2] > 0 and ms[1] + ms[2] <= ms[0] * 1.001
+    assert span > 0
+    a.close()
+    b.close()
