@@ -203,7 +203,7 @@ def bench_ensemble(args, emit=True):
     b = pympm.Batch(cfgs, [w["particles"] for w in ens])
     b.step(args.warmup)
     tile, cap, roles, trial = b.autotune_tiles(trial_steps=4)
-    tile_name = {1: "16x8", 2: "8x4"}[tile]
+    tile_name = {1: "16x8", 2: "8x4", 3: "16x8/512"}[tile]
     log(f"bench: autotuned P2G tiles {tile_name} nodes, bin capacity {cap}, {roles} threads per "
         f"node (ms/step for 16x8 then 8x4, each 16/32/64 x 2/3: {trial.tolist()})")
     nact0 = sum(int(b.active(k).sum()) for k in range(len(ens)))

@@ -143,10 +143,11 @@ typedef struct mpm_hip_config {
                                workgroups than 32x32-node tiles the solve
                                takes its Jacobi-preconditioned path) */
     int p2g_tile;           /* nodes per P2G tile: 0 = chosen from the grid
-                               size, 1 = 16 x 8, 2 = 8 x 4 (no effect on
-                               results; an ensemble takes cfgs[0]'s, 0
-                               meaning 16 x 8, and its autotuner can
-                               switch) */
+                               size, 1 = 16 x 8, 2 = 8 x 4, 3 = 16 x 8
+                               staged in batches of 512 records with four
+                               workgroups per CU (no effect on results; an
+                               ensemble takes cfgs[0]'s, 0 meaning 16 x 8,
+                               and its autotuner can switch) */
 } mpm_hip_config;
 
 enum mpm_hip_scheme {
@@ -328,11 +329,13 @@ int mpm_hip_batch_autotune(mpm_hip_batch *b, int trial_steps, int *chosen, doubl
 int mpm_hip_batch_autotune_launch(mpm_hip_batch *b, int trial_steps, int *capacity,
                                   int *threads_per_node, double *ms);
 /* The search over P2G tile shape as well: 16 x 8-node tiles (the colours
- * staged one after another) and 8 x 4-node tiles (all four colours of a
- * tile staged at once), each with the six pairs above.  *tile = the shape
- * kept, as mpm_hip_config.p2g_tile codes it (1: 16 x 8, 2: 8 x 4);
- * ms[12] = trial time per step, the six pairs of 16 x 8 then of 8 x 4.  The
- * batch's initial shape is cfgs[0].p2g_tile (2: 8 x 4, otherwise 16 x 8).
+ * staged one after another), 8 x 4-node tiles (all four colours of a tile
+ * staged at once) and 16 x 8-node tiles staged in batches of 512 records,
+ * each with the six pairs above.  *tile = the shape kept, as
+ * mpm_hip_config.p2g_tile codes it (1: 16 x 8, 2: 8 x 4, 3: 16 x 8 with
+ * 512-record batches); ms[18] = trial time per step, the six pairs of each
+ * shape in that order.  The batch's initial shape is cfgs[0].p2g_tile (0
+ * meaning 16 x 8).
  * The trajectory does not depend on the choice.  (The reference runs one
  * job per process and has no ensemble; this has no counterpart there.) */
 int mpm_hip_batch_autotune_tiles(mpm_hip_batch *b, int trial_steps, int *tile, int *capacity,

@@ -363,11 +363,12 @@ class Batch:
         return cap.value, roles.value, ms
 
     def autotune_tiles(self, trial_steps=4):
-        """(tile shape as p2g_tile codes it: 1 16 x 8, 2 8 x 4; capacity;
-        threads per node; per-step trial ms [12]: 16 x 8 then 8 x 4, each
-        over capacity 16/32/64 x 2/3 threads per node)."""
+        """(tile shape as p2g_tile codes it: 1 16 x 8, 2 8 x 4, 3 16 x 8 in
+        512-record batches; capacity; threads per node; per-step trial ms
+        [18]: the three shapes in that order, each over capacity 16/32/64 x
+        2/3 threads per node)."""
         tile, cap, roles = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        ms = np.zeros(12, dtype=np.float64)
+        ms = np.zeros(18, dtype=np.float64)
         _check(self.lib.mpm_hip_batch_autotune_tiles(self.b, trial_steps, ctypes.byref(tile),
                                                      ctypes.byref(cap), ctypes.byref(roles),
                                                      _dptr(ms)),

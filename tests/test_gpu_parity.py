@@ -216,13 +216,13 @@ SHAPE_CASES = {
 
 
 @pytest.mark.parametrize("scheme", [pympm.SCHEME_USL, pympm.SCHEME_MUSL])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 3])
 @pytest.mark.parametrize("case", sorted(SHAPE_CASES))
 def test_p2g_tile_shapes_bitwise(case, tile, scheme):
-    """Both P2G tile shapes (16 x 8 nodes, colours staged one by one; 8 x 4
-    nodes, colours merged into shared staging batches), forced through the
-    config on the same inputs, in the USL and MUSL steps: bitwise against
-    the oracle."""
+    """Every P2G tile shape (16 x 8 nodes, colours staged one by one in
+    batches of 256 or of 512 records; 8 x 4 nodes, colours merged into
+    shared staging batches), forced through the config on the same inputs,
+    in the USL and MUSL steps: bitwise against the oracle."""
     w = dict(SHAPE_CASES[case](), scheme=scheme)
     p = w["particles"]
     say(f"case {case}: tile {tile} scheme {scheme} np={len(p['x'])} steps={w['steps']}")
@@ -242,7 +242,7 @@ def test_p2g_tile_shapes_bitwise(case, tile, scheme):
 
 def test_p2g_tile_config_is_validated():
     w = workloads.small_box(cells=16, bx=5, by=9)
-    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"], p2g_tile=3)
+    cfg = pympm.make_config(w["N"], w["h"], w["dt"], material=w["material"], p2g_tile=4)
     with pytest.raises(pympm.MpmHipError):
         pympm.Sim(cfg, w["particles"])
 
