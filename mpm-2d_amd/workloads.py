@@ -67,9 +67,10 @@ def triangle_pile(base_cells, height_cells, n_cells, seed=3, rho=1500.0, ppc_sid
     return {k: v[keep].copy() for k, v in p.items()}
 
 
-# Material parameters named by the workloads (E, nu from the configs; the
-# reference's g_local_mu2 hard-codes the rest: materialsrc/g_local_mu2.c:30-41).
-BASELINE_MATERIAL = dict(E=1e6, nu=0.3, mu_s=0.38, mu_2=0.64, I_0=0.279,
+# Material parameters of the workloads (E = 1e6, nu = 0.3, mu_s = 0.38,
+# mu_2 = 0.64, I_0 = 0.279, d = 1e-3); the reference's g_local_mu2
+# hard-codes its own values of the rest (materialsrc/g_local_mu2.c:30-41).
+WORKLOAD_MATERIAL = dict(E=1e6, nu=0.3, mu_s=0.38, mu_2=0.64, I_0=0.279,
                          grains_d=1e-3, grains_rho=2450.0, rho_crit=1485.0)
 REFERENCE_MATERIAL = dict(E=1e6, nu=0.3, mu_s=0.3819, mu_2=0.6435, I_0=0.278,
                           grains_d=0.001 * 5, grains_rho=2450.0, rho_crit=1485.0)
@@ -82,7 +83,7 @@ def column_collapse(seed=1):
     particles, box with frictionless walls and no-slip floor."""
     n = 64
     return dict(name="column_collapse_64", N=n + 1, h=1.0 / n, dt=1e-5, steps=1000,
-                material=BASELINE_MATERIAL, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
+                material=WORKLOAD_MATERIAL, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
                 particles=block(16, 32, n, seed=seed))
 
 
@@ -92,7 +93,7 @@ def silo(seed=2, cells=512):
     n = cells
     blk = cells // 2
     return dict(name=f"silo_{n}", N=n + 1, h=1.0 / n, dt=2e-6, steps=2000,
-                material=BASELINE_MATERIAL, g_mag=9.81, ramp_time=0.1,
+                material=WORKLOAD_MATERIAL, g_mag=9.81, ramp_time=0.1,
                 bc_kind=2, bc_fp64=(0.05, 0.0, 0.6), bc_int=(),
                 particles=block(blk, blk, n, seed=seed))
 
@@ -149,7 +150,7 @@ def heap(cells=256, seed=3, **kw):
     parts = lithostatic(triangle_pile(cells, cells // 2, cells, seed=seed),
                         lambda x: height * (1.0 - np.abs(x - xc) / half))
     d = dict(name=f"heap_{cells}", N=cells + 1, h=h, dt=5e-6, steps=500,
-             material=BASELINE_MATERIAL, nonlocal_material=HEAP_NONLOCAL,
+             material=WORKLOAD_MATERIAL, nonlocal_material=HEAP_NONLOCAL,
              g_mag=9.81, ramp_time=0.1, **BOX_FRICTION, particles=parts)
     d.update(kw)
     return d
@@ -171,7 +172,7 @@ def ensemble(members=256, seed=11, p_min=2048, p_max=65536, sides=None):
     E = np.exp(np.log(5e5) + uniform01(seed, ids, 2) * (np.log(5e6) - np.log(5e5)))
     out = []
     for k, n in enumerate(sides):
-        mat = dict(BASELINE_MATERIAL, mu_s=float(mu_s[k]), E=float(E[k]))
+        mat = dict(WORKLOAD_MATERIAL, mu_s=float(mu_s[k]), E=float(E[k]))
         out.append(dict(name=f"member_{k}_{n}", N=n + 1, h=1.0 / n, dt=1e-5,
                         material=mat, g_mag=9.81, ramp_time=0.1, **BOX_FRICTION,
                         particles=block(n // 4, n // 2, n, seed=seed * 1000 + k)))
@@ -191,8 +192,8 @@ def adjoint_block(seed=4, cells=128, material=None):
     p = lithostatic(p, lambda x: top + 0.0 * x, rho=1500.0, g=9.81, k0=0.4)
     p["vx"] = np.full(len(p["x"]), 0.5)
     return dict(name=f"adjoint_block_{n}", N=n + 1, h=1.0 / n, dt=1e-5, steps=200,
-                material=material or dict(BASELINE_MATERIAL), g_mag=9.81, ramp_time=1e-9,
+                material=material or dict(WORKLOAD_MATERIAL), g_mag=9.81, ramp_time=1e-9,
                 **BOX_FRICTION, particles=p)
 
 
-ADJOINT_TARGET = dict(BASELINE_MATERIAL, mu_s=0.42, E=2e6)
+ADJOINT_TARGET = dict(WORKLOAD_MATERIAL, mu_s=0.42, E=2e6)

@@ -37,7 +37,7 @@ def small_case(cells=16, seed=3):
     p["vx"] = 0.05 * rng.standard_normal(len(p["x"]))
     p["vy"] = 0.05 * rng.standard_normal(len(p["x"]))
     return dict(name="adjoint_small", N=n + 1, h=1.0 / n, dt=1e-5,
-                material=dict(workloads.BASELINE_MATERIAL), g_mag=9.81, ramp_time=0.1,
+                material=dict(workloads.WORKLOAD_MATERIAL), g_mag=9.81, ramp_time=0.1,
                 **workloads.BOX_FRICTION, particles=p)
 
 
@@ -102,7 +102,7 @@ def test_material_reverse_branches():
     """Closed forms of the branches: the density cut-off passes nothing back;
     an elastic update is linear in the stress, so its adjoint of T is the
     transpose of the (rotation) map and carries no mu_s dependence."""
-    c = AO.constants(workloads.BASELINE_MATERIAL)
+    c = AO.constants(workloads.WORKLOAD_MATERIAL)
     one = np.ones(1)
     T = [np.array([-100.0]), np.array([3.0]), np.array([-120.0]), np.array([-90.0])]
     Tb = [np.array([1.0]), np.array([0.5]), np.array([-2.0]), np.array([0.25])]

@@ -100,7 +100,7 @@ def test_one_step_bitwise_all_fields():
 CASES = {
     "box_reference_material": lambda: workloads.small_box(cells=16, bx=5, by=9, steps=100),
     "box_baseline_material": lambda: workloads.small_box(cells=32, bx=9, by=20, steps=100,
-                                                         material=workloads.BASELINE_MATERIAL,
+                                                         material=workloads.WORKLOAD_MATERIAL,
                                                          g_mag=9.81),
     "silo_64": lambda: dict(workloads.silo(seed=5, cells=64), steps=100),
 }

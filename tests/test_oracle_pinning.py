@@ -388,8 +388,8 @@ def _stress(lib, T, m, v, gp, L, dt, q):
 
 def test_g_local_mu2_follows_mu_of_I():
     lib = pyoracle.load()
-    q = pyoracle.make_params(lib, 9, 0.125, 1e-5, material=workloads.BASELINE_MATERIAL)
-    mat = workloads.BASELINE_MATERIAL
+    q = pyoracle.make_params(lib, 9, 0.125, 1e-5, material=workloads.WORKLOAD_MATERIAL)
+    mat = workloads.WORKLOAD_MATERIAL
     E, nu = mat["E"], mat["nu"]
     G = E / (2 * (1 + nu))
     rng = np.random.default_rng(8)
