@@ -16,6 +16,7 @@ typedef struct lexer {
     size_t pos, len;
     const char *path;
     int line;
+    int open_comment;   /* a block comment ran to the end of the text */
 } lexer;
 
 enum tok { T_END, T_WORD, T_STRING, T_LBRACE, T_RBRACE, T_EQ, T_COMMA, T_LPAREN, T_RPAREN,
@@ -37,6 +38,11 @@ static void skip_space(lexer *lx)
             while (lx->pos + 1 < lx->len && !(lx->s[lx->pos] == '*' && lx->s[lx->pos + 1] == '/')) {
                 if (lx->s[lx->pos] == '\n') lx->line++;
                 lx->pos++;
+            }
+            if (lx->pos + 1 >= lx->len) {
+                lx->open_comment = 1;
+                lx->pos = lx->len;
+                return;
             }
             lx->pos += 2;
         } else {
@@ -68,6 +74,7 @@ static enum tok next(lexer *lx, char **text)
 {
     *text = NULL;
     skip_space(lx);
+    if (lx->open_comment) return T_ERR;
     if (lx->pos >= lx->len) return T_END;
     char c = lx->s[lx->pos];
     switch (c) {
@@ -161,7 +168,8 @@ static int read_items(cfg_file *cfg, lexer *lx, const char *section, int depth)
         }
         if (t != T_WORD) {
             free(word);
-            return fail(cfg, lx, "expected an option name");
+            return fail(cfg, lx, lx->open_comment ? "unterminated comment"
+                                                  : "expected an option name");
         }
         char *val;
         enum tok t2 = next(lx, &val);
@@ -242,7 +250,7 @@ static int read_text(cfg_file *cfg, const char *path, const char *section, int d
     size_t got = fread(buf, 1, (size_t)n, fp);
     fclose(fp);
     buf[got] = 0;
-    lexer lx = {buf, 0, got, path, 1};
+    lexer lx = {buf, 0, got, path, 1, 0};
     int rc = read_items(cfg, &lx, section, depth);
     free(buf);
     return rc;
@@ -251,7 +259,15 @@ static int read_text(cfg_file *cfg, const char *path, const char *section, int d
 int cfg_read(cfg_file *cfg, const char *path)
 {
     memset(cfg, 0, sizeof *cfg);
-    return read_text(cfg, path, "", 0);
+    const int rc = read_text(cfg, path, "", 0);
+    if (rc) {
+        /* a file that does not parse leaves no entries, only the message */
+        char error[sizeof cfg->error];
+        memcpy(error, cfg->error, sizeof error);
+        cfg_release(cfg);
+        memcpy(cfg->error, error, sizeof error);
+    }
+    return rc;
 }
 
 void cfg_release(cfg_file *cfg)

@@ -488,7 +488,10 @@ int main(int argc, char **argv)
     if (cfg_mode && o.check_cfg) {
         /* print the settings the run would use, without running */
         int rc = apply_cfg(&cf, &o, NULL);
-        if (rc) return rc;
+        if (rc) {
+            cfg_release(&cf);
+            return rc;
+        }
         printf("particle-file=%s\ngrid-file=%s\nmaterial-file=%s\nbc-file=%s\n", o.pfile,
                o.grid, o.mat, o.bc);
         printf("dt=%.17g\nautomatic-dt=%ld\nsample-rate=%.17g\n",
