@@ -24,7 +24,9 @@ contraction) on the host's cores over a bounded sample of the same
 workload.  The sub-records `batched` and `adjoint` (with
 `batched_vs_plain` and `adjoint_vs_plain`) measure the 256-member ensemble
 against the same members as single jobs and the 200-step adjoint against
-its plain forward run, each with its own accuracy check.
+its plain forward run, each with its own accuracy check; `nonlocal` times
+100 steps of the g_nonlocal fluidity heap (256x256 cells, 262,144
+particles) with its CG iterations and final residual.
 """
 import argparse
 import json
@@ -130,7 +132,7 @@ def cpu_baseline(w, budget_s=15.0, variants=True):
     return out
 
 
-def bench_heap(args):
+def bench_heap(args, emit=True):
     """The nonlocal fluidity heap (g_nonlocal, 256x256 cells, 262,144
     particles): the same step plus the per-step fluidity assembly, a
     preconditioned conjugate-gradient solve to |r| <= 1e-10 |f| and the
@@ -182,7 +184,9 @@ def bench_heap(args):
         log("bench: CPU baseline (oracle restatement with a banded Cholesky solve)")
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
-    print(json.dumps(result), flush=True)
+    if emit:
+        print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_ensemble(args, emit=True):
@@ -515,7 +519,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU-baseline leg (and with it the accuracy gate)")
     ap.add_argument("--no-subrecords", action="store_true",
-                    help="silo only: no batched-ensemble and adjoint sub-records")
+                    help="silo only: no batched-ensemble, adjoint and nonlocal-heap sub-records")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--repeats", type=int, default=3, help="adjoint workload: timed runs")
     ap.add_argument("--workload", choices=["silo", "heap", "ensemble", "adjoint"],
@@ -559,6 +563,12 @@ def main():
         result["adjoint"] = {k: adj[k] for k in ("value", "unit", "forward_ms", "reverse_ms",
                                                  "plain_forward_ms", "config", "accuracy",
                                                  "calibration", "cpu_baseline")}
+        log("bench: sub-record: the g_nonlocal fluidity heap")
+        sub.steps, sub.warmup, sub.no_cpu_baseline = 100, 20, True
+        heap = bench_heap(sub, emit=False)
+        result["nonlocal"] = {k: heap[k] for k in ("value", "unit", "ms_per_step", "config",
+                                                   "cg_iterations_mean", "cg_iterations_max",
+                                                   "cg_final_rel_residual")}
     print(json.dumps(result), flush=True)
 
 
